@@ -1,0 +1,127 @@
+/*
+ * C ABI of libsbewh_hip.so: the MI355X (gfx950) implementation of the SBE
+ * right-hand side, its adaptive Dormand-Prince RK5(4) propagation and the
+ * current reduction.  Plain pointers and sizes only.
+ *
+ * State vectors use the reference's CPU layout (code/src/StateContext.h:7-25):
+ * complex entries, interleaved (re, im) doubles,
+ *   [A_x, A_y, A_z, 5 zero pads][rho_k0 (n*n, row-major, zero padded to P)][rho_k1]...
+ * with P = 8 * ceil(n*n / 8) and k in Fourier-block order
+ * (code/src/WhFourierTransform.cpp:137-149, 303-312).  Length
+ * 8 + P * N_k complex entries (sbe_state_size).
+ *
+ * Every function returns 0 on success and a negative code on error; the
+ * message is available from sbe_last_error(ctx) (or sbe_last_error(NULL) for
+ * failures of sbe_create).  Nothing throws across the ABI.  A context owns
+ * all device memory it uses and one HIP stream; use one context per host
+ * thread.  Calls are synchronous unless stated otherwise.
+ */
+#ifndef SBEWH_HIP_H
+#define SBEWH_HIP_H
+
+#include "sbewh_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sbe_ctx sbe_ctx;
+
+enum sbe_status {
+    SBE_OK = 0,
+    SBE_ERR_ARG = -1,        /* invalid argument / unsupported configuration */
+    SBE_ERR_HIP = -2,        /* HIP runtime or launch failure */
+    SBE_ERR_STEP = -3,       /* step-size control failed (500 rejected tries) */
+    SBE_ERR_PHYSICS = -4,    /* e.g. no unique band gap for the Fermi level */
+    SBE_ERR_REPLAY = -5      /* replay log exhausted / inconsistent */
+};
+
+const char *sbe_version(void);
+const char *sbe_last_error(const sbe_ctx *ctx);
+
+/* Construction from the real-space model, grid, parameters and pulse.
+ * Replaces PropagatorCPU(const StateContext&, WhFourierTransform&)
+ * (code/src/PropagatorCPU.h:52) and CuPropagator(cudaStream_t, ...)
+ * (code/src/CuPropagator.h:31) together with the WhFourierTransform /
+ * CuWhFourierTransformHk setup (code/src/WhFourierTransform.cpp:3-38,
+ * code/src/CuWhFourierTransform.cu:3-40).  Applies the reference's Nf rule
+ * (code/src/Parameter.cpp:4-56) and dimensional projection
+ * (code/src/util/w90util.cpp:417-444).  Supports n <= 16 bands. */
+int sbe_create(const sbe_model *model, const sbe_grid *grid, const sbe_params *params,
+               const sbe_subpulse *pulses, int num_pulses, sbe_ctx **out);
+void sbe_destroy(sbe_ctx *ctx);
+
+/* Replace the pulse (the reference's multi-configuration loop assigns
+ * param.pulse per configuration, code/src/main/runSBE.cpp:205-216). */
+int sbe_set_pulses(sbe_ctx *ctx, const sbe_subpulse *pulses, int num_pulses);
+
+long sbe_state_size(const sbe_ctx *ctx);   /* complex entries */
+long sbe_num_k(const sbe_ctx *ctx);
+int sbe_get_nf(const sbe_ctx *ctx, int nf[3]);
+/* pulse window [t0, t1] (code/src/pulse/Pulse3D.cpp:42-62) */
+int sbe_time_window(const sbe_ctx *ctx, double *t0, double *t1);
+/* E(t) of the pulse (Pulse3D::getE, code/src/pulse/Pulse3D.cpp:15-21) */
+int sbe_pulse_E(const sbe_ctx *ctx, double t, double E[3]);
+
+/* Fermi level from all eigenvalues at A = 0 (getFermiLevelCPU / calcFermiLevelBase,
+ * code/src/main/runSBE.cpp:97-155).  below_gap != 0 selects occupiedBelowBandGap. */
+int sbe_fermi_level(sbe_ctx *ctx, int occupied_bands, int below_gap, double *mu);
+
+/* Initial Fermi-Dirac state, host buffer of sbe_state_size complex entries
+ * (PropagatorCPU::calcInitialState, code/src/PropagatorCPU.h:54;
+ *  CuPropagator::calcInitialState, code/src/CuPropagator.h:33). */
+int sbe_initial_state(sbe_ctx *ctx, double mu, double *state);
+
+/* The RHS functor on host buffers: dsdt = f(state, t), dsdt fully overwritten.
+ * Replaces void PropagatorCPU::operator()(const CPUstate&, CPUstate&, double)
+ * (code/src/PropagatorCPU.h:55, body :208-256). */
+int sbe_rhs(sbe_ctx *ctx, const double *state, double *dsdt, double t);
+
+/* The same on device buffers, enqueued on `stream` (a hipStream_t; NULL = the
+ * context's stream) and returning without synchronising.  Replaces
+ * CuPropagator::operator()(const GPUstate&, GPUstate&, double)
+ * (code/src/CuPropagator.h:34). */
+int sbe_rhs_device(sbe_ctx *ctx, const double *d_state, double *d_dsdt, double t, void *stream);
+
+/* Current j(t) of a host state (ObserverCPU::operator() with the default
+ * region, code/src/ObserverCPU.h:43, code/src/ObserverCPU.cpp:149-210,293-308). */
+int sbe_observe(sbe_ctx *ctx, const double *state, double j[3]);
+
+/* Whole propagation, device resident:
+ * integrate_const(make_controlled<runge_kutta_dopri5>(eps_abs, eps_rel), rhs, state,
+ *                 t0, t1 + dt_out/2, dt_out, observer)
+ * (code/src/main/runSBE.cpp:170-183, code/src/main/CuRunSBE.cu:19-31) with
+ * dt_out = (t1 - t0)/(output_count - 1).  state (host) is advanced in place.
+ * t_out[output_count], j_out[output_count][3] receive the observer output
+ * (either may be NULL).  log (optional) receives every step attempt.  If
+ * replay is non-NULL its accepted entries dictate the step sizes.  Returns
+ * the number of accepted steps (>= 0) or an error code; rhs_calls (optional)
+ * receives the number of RHS evaluations. */
+long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out, double *j_out,
+                   sbe_steplog *log, const sbe_steplog *replay, long *rhs_calls);
+
+/* Kernel-level entry points (used by tests and the benchmark). */
+
+/* H0(k + A) and H_E(k + A) for every k, out[k][2][n*n] complex (the
+ * transformMeshFFT(HkVec...) product, code/src/WhFourierTransform.cpp:259-315). */
+int sbe_hk_mesh(sbe_ctx *ctx, const double A[3], const double E[3], double *out);
+
+/* Batched Jacobi on `count` Hermitian n x n matrices H (host, complex):
+ * U (count x n x n) with U H U^dagger = diag(eig)  (DiagonalizatorSingle::diag,
+ * code/src/util/DiagonalizatorSingle.cpp:99-122; CuDiagonalizator::diag,
+ * code/src/CuDiagonalizator.cu:45-51). */
+int sbe_jacobi_batch(int n, int sweeps, long count, const double *H, double *U, double *eig);
+
+/* Device-resident timing of `steps` Dormand-Prince step attempts at fixed
+ * (t, dt) from the given host state (each attempt = 6 RHS evaluations,
+ * stage algebra and error norm; the state is not advanced).  ms receives the
+ * wall time of the timed attempts, stage_ms the summed duration of the fused
+ * stage kernel measured with HIP events (may be NULL). */
+int sbe_bench_attempts(sbe_ctx *ctx, const double *state, double t, double dt, int warmup, int steps,
+                       double *ms, double *stage_ms, double *phase_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,104 @@
+/*
+ * Plain-C data types shared by the HIP library (libsbewh_hip.so) and the CPU
+ * oracle.  Every quantity is in Hartree atomic units; complex numbers are
+ * interleaved (re, im) doubles.  No torch or HIP types appear here.
+ *
+ * The structs carry what the reference keeps in Parameter_t and
+ * TightBindingParameter_t for the propagation path:
+ *   - sbe_model   <- TightBindingParameter_t::realSpaceOps / latticeVectors
+ *                    (reference: code/src/io/TightBindingParameter.h,
+ *                     code/src/util/w90util.h W90_realSpaceOperators)
+ *   - sbe_grid    <- PropagationParameter_t::Nk, kGlobalShift,
+ *                    FourierTransformParameter_t::Nf, GeneralParameter dim
+ *                    (code/src/Parameter.cpp:4-56)
+ *   - sbe_params  <- PropagationParameter_t (relaxationF2, soothingWidth,
+ *                    epsAbs, epsRel), DiagonalizationParameter_t
+ *                    (maxSweepCount), TightBindingParameter_t::gsTemp
+ *                    (code/src/io/PropagationParameter.cpp:65-111)
+ *   - sbe_subpulse <- one Pulse1D + polarization + centre inside Pulse3D
+ *                    (code/src/pulse/Pulse3D.cpp:8-21,
+ *                     code/src/io/PulseParameter.cpp:121-162)
+ */
+#ifndef SBEWH_TYPES_H
+#define SBEWH_TYPES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Real-space tight-binding operators.  H[c] is the n x n Hamiltonian block of
+ * cell c (row-major, complex), D[c][d] the position-operator block r_d(R_c).
+ * cells[c] holds the integer lattice-vector indices (R_0, R_1, R_2). */
+typedef struct sbe_model {
+    int num_wann;             /* n: number of Wannier functions (bands) */
+    int num_cells;            /* N_R */
+    const int *cells;         /* [num_cells][3] */
+    const double *H;          /* [num_cells][n][n][2] */
+    const double *D;          /* [num_cells][3][n][n][2] */
+    double lattice[9];        /* row i = lattice vector a_i (bohr) */
+} sbe_model;
+
+/* Simulation k-grid.  Nf is the Fourier block size that fixes the k-point
+ * order of the state vector (k index = c + Nf2*(b + Nf1*(a + Nf0*block)),
+ * see code/src/WhFourierTransform.cpp:303-312).  Nf[d] == 0 selects the
+ * reference's automatic rule (extent of the model, rounded up to a product
+ * of primes <= 5). */
+typedef struct sbe_grid {
+    int dim;                  /* 1, 2 or 3 */
+    int Nk[3];
+    int Nf[3];
+    double k_global_shift[3]; /* fractional coordinates */
+} sbe_grid;
+
+/* Propagation / relaxation / eigensolver parameters. */
+typedef struct sbe_params {
+    double relaxation_F2;     /* 1/T2; <= 0 disables dephasing */
+    double soothing_width;    /* w; < 0 selects the (a != b) dephasing mask */
+    double gs_temp;           /* k_B T of the initial Fermi-Dirac state */
+    double eps_abs;           /* RK45 absolute tolerance */
+    double eps_rel;           /* RK45 relative tolerance */
+    int sweep_count;          /* Jacobi sweeps per diagonalisation */
+    int reserved;
+} sbe_params;
+
+enum sbe_pulse_type {
+    SBE_PULSE_SIN2 = 0,       /* code/src/pulse/Sin2Pulse1D.cpp */
+    SBE_PULSE_GAUSS = 1,      /* code/src/pulse/GaussianPulse1D.cpp */
+    SBE_PULSE_PURE_GAUSS = 2, /* code/src/pulse/PureGauss1D.cpp */
+    SBE_PULSE_SIN2_RAMP = 3   /* code/src/pulse/Sin2RampPulse1D.cpp */
+};
+
+/* One linearly polarised sub-pulse, already in atomic units:
+ * E0 (field amplitude), omega (carrier), fwhm (envelope parameter; for sin2
+ * the field vanishes outside |t| > fwhm), t_start_fwhm / t_end_fwhm (window in
+ * units of fwhm), center (time shift).  pol need not be normalised. */
+typedef struct sbe_subpulse {
+    int type;
+    int cycles_on;            /* sin2Ramp only */
+    double E0;
+    double omega;
+    double cep;
+    double fwhm;
+    double t_start_fwhm;
+    double t_end_fwhm;
+    double center;
+    double rising_cycles;     /* sin2Ramp only */
+    double pol[3];
+} sbe_subpulse;
+
+/* Record of the adaptive controller: one entry per step attempt. */
+typedef struct sbe_steplog {
+    int capacity;             /* entries the arrays can hold */
+    int count;                /* entries written (may exceed capacity: then
+                                 only the first `capacity` are stored) */
+    double *t;                /* time at the start of the attempt */
+    double *dt;               /* step size tried */
+    double *err;              /* max relative error of the attempt */
+    int *accepted;            /* 1 accepted, 0 rejected */
+} sbe_steplog;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,70 @@
+"""ctypes loader for libsbewh_hip.so (the C ABI in include/sbewh_hip.h).
+
+There is no CPU fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .ctypes_types import SbeModel, SbeGrid, SbeParams, SbeSubpulse, SbeSteplog
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsbewh_hip.so")
+
+# every symbol include/sbewh_hip.h declares
+EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_set_pulses", "sbe_state_size",
+           "sbe_num_k", "sbe_get_nf", "sbe_time_window", "sbe_pulse_E", "sbe_fermi_level", "sbe_initial_state",
+           "sbe_rhs", "sbe_rhs_device", "sbe_observe", "sbe_integrate", "sbe_hk_mesh", "sbe_jacobi_batch",
+           "sbe_bench_attempts")
+
+_lib = None
+
+
+class SbeError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (built by ``python -m sbewh_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SbeError(f"{LIB_PATH} is missing: build it with `python -m sbewh_amd.build` "
+                       "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, vp, d, i, l = C.POINTER, C.c_void_p, C.c_double, C.c_int, C.c_long
+    dp = P(C.c_double)
+    L.sbe_version.restype = C.c_char_p
+    L.sbe_last_error.restype = C.c_char_p
+    L.sbe_last_error.argtypes = [vp]
+    L.sbe_create.argtypes = [P(SbeModel), P(SbeGrid), P(SbeParams), P(SbeSubpulse), i, P(vp)]
+    L.sbe_destroy.argtypes = [vp]
+    L.sbe_set_pulses.argtypes = [vp, P(SbeSubpulse), i]
+    L.sbe_state_size.restype = l
+    L.sbe_state_size.argtypes = [vp]
+    L.sbe_num_k.restype = l
+    L.sbe_num_k.argtypes = [vp]
+    L.sbe_get_nf.argtypes = [vp, P(C.c_int)]
+    L.sbe_time_window.argtypes = [vp, dp, dp]
+    L.sbe_pulse_E.argtypes = [vp, d, dp]
+    L.sbe_fermi_level.argtypes = [vp, i, i, dp]
+    L.sbe_initial_state.argtypes = [vp, d, dp]
+    L.sbe_rhs.argtypes = [vp, dp, dp, d]
+    L.sbe_rhs_device.argtypes = [vp, vp, vp, d, vp]
+    L.sbe_observe.argtypes = [vp, dp, dp]
+    L.sbe_integrate.restype = l
+    L.sbe_integrate.argtypes = [vp, dp, i, dp, dp, P(SbeSteplog), P(SbeSteplog), P(C.c_long)]
+    L.sbe_hk_mesh.argtypes = [vp, dp, dp, dp]
+    L.sbe_jacobi_batch.argtypes = [i, i, l, dp, dp, dp]
+    L.sbe_bench_attempts.argtypes = [vp, dp, d, d, i, i, dp, dp, dp]
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None):
+    if rc < 0:
+        msg = load().sbe_last_error(ctx).decode()
+        raise SbeError(f"sbewh error {rc}: {msg}")
+    return rc

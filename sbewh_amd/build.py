@@ -1,0 +1,71 @@
+"""Build libsbewh_hip.so for gfx950 with hipcc (in-tree: sbewh_amd/lib/).
+
+    python -m sbewh_amd.build          # incremental
+    python -m sbewh_amd.build --force  # rebuild everything
+"""
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libsbewh_hip.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# -packed-fp32-ops: keep the compiler from forming packed FP32 instructions
+COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{SRC}"]
+SOURCES = ["phase_kernels.hip", "stage_kernels.hip", "rk_kernels.hip", "observer_kernels.hip", "sbe_api.cpp"]
+HEADERS = ["device_common.hpp", "kernels.hpp"]
+
+
+def _newer(a: str, deps: list) -> bool:
+    if not os.path.exists(a):
+        return False
+    t = os.path.getmtime(a)
+    return all(os.path.getmtime(d) <= t for d in deps)
+
+
+def _compile(src: str, force: bool) -> str:
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    deps = [os.path.join(SRC, src)] + [os.path.join(SRC, h) for h in HEADERS] + \
+        [os.path.join(INCLUDE, "sbewh_hip.h"), os.path.join(INCLUDE, "sbewh_types.h")]
+    if not force and _newer(obj, deps):
+        return obj
+    lang = ["-x", "hip"]
+    # the host controller's time arithmetic is kept free of FMA contraction so
+    # that output times are the plain IEEE sums t0 + step * dt_out
+    extra = ["-ffp-contract=off"] if src.endswith(".cpp") else []
+    cmd = [HIPCC] + COMMON + extra + lang + ["-c", os.path.join(SRC, src), "-o", obj + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+    os.replace(obj + ".tmp", obj)
+    return obj
+
+
+def build(force: bool = False, verbose: bool = True) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
+    if force or not _newer(LIB, objs):
+        cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", LIB + ".tmp"] + objs
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stderr}")
+        os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(force="--force" in sys.argv)

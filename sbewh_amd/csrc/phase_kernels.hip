@@ -1,0 +1,185 @@
+// Wannier -> Bloch phase sum X(k) = sum_R exp(i 2 pi k.R) X(R) on CDNA4, FP64.
+//
+// The reference evaluates it as pre-phase + batched FFT + post-phase
+// (code/src/WhFourierTransform.cpp:208-315; GPU: code/src/CuWhFourierTransform.cu:96-173).
+// Here the k + A shift is folded into per-dimension twiddles
+//   tw_d[i][r] = exp(i 2 pi (i/Nk_d + off_d) (r + rmin_d)),  off = frac(kGlobalShift + A),
+// and the sum over the dense R box is contracted one axis at a time
+// (R_2 -> i_2, R_1 -> i_1, R_0 -> i_0), i.e. a chain of small complex GEMMs
+// [Nk_d x W_d] x [W_d x (rest)].  The last contraction scatters into the
+// state's k order through a precomputed grid->k map.  The operator merge
+// H_E(R) = H(R) + sum_d E_d r_d(R) (code/src/WhFourierTransform.cpp:63-66) happens
+// while the first contraction reads the model.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace sbe {
+
+__global__ void twiddle_kernel(PhaseGeom g, const double *a_src, int use_shift, double *tw)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int n0 = g.Nk[0] * g.W[0], n1 = g.Nk[1] * g.W[1], n2 = g.Nk[2] * g.W[2];
+    if (t >= n0 + n1 + n2) return;
+    int d = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
+    int local = t - (d == 0 ? 0 : (d == 1 ? n0 : n0 + n1));
+    int i = local / g.W[d], r = local % g.W[d];
+    double k_au[3];
+    for (int e = 0; e < 3; e++) {
+        double A = (a_src && e < g.dim) ? a_src[2 * e] : 0.0;
+        k_au[e] = (use_shift ? g.kshift_au[e] : 0.0) + A;
+    }
+    // toFracK: (k . a_d) / 2 pi  (code/src/WhFourierTransform.cpp:46-52)
+    const double *ad = g.lattice + 3 * d;
+    double off = (k_au[0] * ad[0] + k_au[1] * ad[1] + k_au[2] * ad[2]) / (2 * M_PI);
+    double ph = ((double)i / g.Nk[d] + off) * (double)(r + g.rmin[d]);
+    double s, c;
+    sincospi(2 * ph, &s, &c);
+    tw[2 * t] = c;
+    tw[2 * t + 1] = s;
+}
+
+struct ContractArgs {
+    const double *in;    // mode 0: [outer][W][inner][M]
+    double *out;         // out mode 0: [outer][Nkd][inner][M]; 1: [k][M] via kmap
+    const double *tw;    // [Nkd][W]
+    int W, Nkd;
+    long outer, inner;
+    int M, nn, in_mode, out_mode, dim;
+    long nbox;
+    const double *Hb, *Db, *dHb;
+    double E0, E1, E2;
+    const int *kmap;
+};
+
+constexpr int IB = 8;
+
+__device__ __forceinline__ cd model_value(const ContractArgs &a, long b, int m)
+{
+    int op = m / a.nn, e = m - op * a.nn;
+    const double *h = a.Hb + 2 * (b * a.nn + e);
+    cd v = mk(h[0], h[1]);
+    if (a.in_mode == 1) {
+        if (op == 1) {
+            double E[3] = {a.E0, a.E1, a.E2};
+            for (int d = 0; d < a.dim; d++) {
+                const double *p = a.Db + 2 * (((long)d * a.nbox + b) * a.nn + e);
+                v.x += E[d] * p[0];
+                v.y += E[d] * p[1];
+            }
+        }
+        return v;
+    }
+    // observer operators: H0, D_0..D_{dim-1}, dH_0..dH_{dim-1}
+    if (op == 0) return v;
+    if (op <= a.dim) {
+        const double *p = a.Db + 2 * (((long)(op - 1) * a.nbox + b) * a.nn + e);
+        return mk(p[0], p[1]);
+    }
+    const double *p = a.dHb + 2 * (((long)(op - 1 - a.dim) * a.nbox + b) * a.nn + e);
+    return mk(p[0], p[1]);
+}
+
+__global__ __launch_bounds__(256) void contract_kernel(ContractArgs a)
+{
+    const int nIB = (a.Nkd + IB - 1) / IB;
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = a.outer * nIB * a.inner * a.M;
+    if (tid >= total) return;
+    int m = (int)(tid % a.M);
+    long rest = tid / a.M;
+    long in = rest % a.inner;
+    rest /= a.inner;
+    int ib = (int)(rest % nIB);
+    long o = rest / nIB;
+    int i0 = ib * IB;
+    cd acc[IB];
+#pragma unroll
+    for (int q = 0; q < IB; q++) acc[q] = mk(0, 0);
+    for (int r = 0; r < a.W; r++) {
+        long src = (o * a.W + r) * a.inner + in;
+        cd x;
+        if (a.in_mode == 0) {
+            const double *p = a.in + 2 * (src * a.M + m);
+            x = mk(p[0], p[1]);
+        } else {
+            x = model_value(a, src, m);
+        }
+#pragma unroll
+        for (int q = 0; q < IB; q++) {
+            int i = i0 + q;
+            if (i < a.Nkd) {
+                const double *t = a.tw + 2 * ((long)i * a.W + r);
+                acc[q] = cfma(mk(t[0], t[1]), x, acc[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < IB; q++) {
+        int i = i0 + q;
+        if (i >= a.Nkd) break;
+        long dst = (o * a.Nkd + i) * a.inner + in;
+        if (a.out_mode == 1) dst = a.kmap[dst];
+        double *p = a.out + 2 * (dst * a.M + m);
+        p[0] = acc[q].x;
+        p[1] = acc[q].y;
+    }
+}
+
+hipError_t launch_twiddles(const PhaseGeom &g, const double *a_src, int use_shift, double *tw, hipStream_t s)
+{
+    int tot = g.Nk[0] * g.W[0] + g.Nk[1] * g.W[1] + g.Nk[2] * g.W[2];
+    hipLaunchKernelGGL(twiddle_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, g, a_src, use_shift, tw);
+    return hipGetLastError();
+}
+
+// stage d runs iff it changes the data: W_d > 1 or Nk_d > 1 (stage 0 always runs)
+static bool stage_runs(const PhaseGeom &g, int d) { return d == 0 || g.W[d] > 1 || g.Nk[d] > 1; }
+
+hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
+                            double E2, double *mesh, hipStream_t s)
+{
+    const int nn = g.n * g.n;
+    ContractArgs a{};
+    a.M = ops * nn;
+    a.nn = nn;
+    a.dim = g.dim;
+    a.nbox = (long)g.W[0] * g.W[1] * g.W[2];
+    a.Hb = b.Hb;
+    a.Db = b.Db;
+    a.dHb = b.dHb;
+    a.E0 = g.dim > 0 ? E0 : 0;
+    a.E1 = g.dim > 1 ? E1 : 0;
+    a.E2 = g.dim > 2 ? E2 : 0;
+    a.kmap = b.kmap;
+    const double *src = nullptr;
+    bool first = true;
+    double *bufs[2] = {b.y2, b.y1};
+    int nb = 0;
+    long twoff[3] = {0, (long)g.Nk[0] * g.W[0], (long)g.Nk[0] * g.W[0] + (long)g.Nk[1] * g.W[1]};
+    for (int d = 2; d >= 0; d--) {
+        if (!stage_runs(g, d)) continue;
+        long outer = 1, inner = 1;
+        for (int e = 0; e < d; e++) outer *= g.W[e];
+        for (int e = d + 1; e < 3; e++) inner *= g.Nk[e];
+        a.in = src;
+        a.in_mode = first ? (ops == 2 ? 1 : 2) : 0;
+        a.W = g.W[d];
+        a.Nkd = g.Nk[d];
+        a.outer = outer;
+        a.inner = inner;
+        a.tw = b.tw + 2 * twoff[d];
+        bool last = (d == 0);
+        a.out_mode = last ? 1 : 0;
+        a.out = last ? mesh : bufs[nb];
+        long total = outer * ((g.Nk[d] + IB - 1) / IB) * inner * a.M;
+        hipLaunchKernelGGL(contract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        src = a.out;
+        nb ^= 1;
+        first = false;
+    }
+    return hipSuccess;
+}
+
+}  // namespace sbe

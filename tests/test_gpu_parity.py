@@ -1,0 +1,132 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances: FP64 throughout.  Single kernels: <= 1e-12 relative.  Whole
+propagations replaying the oracle's accepted step sequence: j(t) and the final
+density matrices <= 1e-10 relative (the target stated for this path).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from sbewh_amd.propagator import SBEContext, jacobi_batch
+from oracle.oracle_py import Oracle
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6, 8, 12, 16])
+def test_jacobi_batch(n):
+    say(f"[gpu] jacobi n={n}")
+    rng = np.random.default_rng(100 + n)
+    H = cases.random_hermitian(rng, 257, n)
+    sweeps = 12
+    U, e = jacobi_batch(H, sweeps)
+    I = np.eye(n)
+    assert np.abs(U @ np.conj(np.swapaxes(U, 1, 2)) - I).max() < 1e-13
+    D = U @ H @ np.conj(np.swapaxes(U, 1, 2))
+    off = D - np.einsum("kii->ki", D)[:, :, None] * I
+    assert np.abs(off).max() < 1e-12 * np.abs(H).max()
+    assert np.allclose(np.einsum("kii->ki", D).real, e, rtol=0, atol=1e-12)
+    ref = np.linalg.eigvalsh(H)
+    assert np.abs(np.sort(e, 1) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name", ["chain", "square", "cubic"])
+def test_phase_sum(name):
+    say(f"[gpu] phase sum {name}")
+    cfg = {"chain": cases.chain(), "square": cases.small_square(), "cubic": cases.small_cubic()}[name]
+    g = SBEContext(cfg)
+    o = Oracle(cfg, "direct")
+    A, E = (0.11, -0.07, 0.05), (0.003, 0.002, -0.001)
+    m_gpu = g.hk_mesh(A, E)
+    m_cpu = o.hk_mesh(A, E)
+    assert g.nf == o.nf
+    assert rel(m_gpu, m_cpu) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["chain", "square", "cubic"])
+def test_fermi_initial_rhs_observe(name):
+    say(f"[gpu] fermi/initial/rhs/observe {name}")
+    cfg = {"chain": cases.chain(sweeps=8), "square": cases.small_square(), "cubic": cases.small_cubic()}[name]
+    g = SBEContext(cfg)
+    o = Oracle(cfg)
+    mu_g = g.fermi_level(cfg.occupied_bands)
+    mu_o = o.fermi_level(cfg.occupied_bands)
+    assert abs(mu_g - mu_o) < 1e-12 * max(1.0, abs(mu_o))
+    st_g = g.initial_state(mu_o)
+    st_o = o.initial_state(mu_o)
+    assert rel(st_g, st_o) < 1e-12
+    x = cases.perturbed_state(st_o, o.n, o.P, o.num_k, seed=3)
+    t = 0.31 * o.t_start
+    d_g = g.rhs(x, t)
+    d_o = o.rhs(x, t)
+    assert rel(d_g, d_o) < 1e-12, rel(d_g, d_o)
+    j_g = g.observe(x)
+    j_o = o.observe_j(x)
+    assert np.abs(j_g - j_o).max() < 1e-12 * max(np.abs(j_o).max(), 1e-30)
+
+
+def test_rhs_no_dephasing_and_step_mask():
+    say("[gpu] rhs variants")
+    import dataclasses
+    base = cases.small_square()
+    for T2, w in ((-1.0, 0.025), (10.0, -1.0)):
+        cfg = dataclasses.replace(base, T2_fs=T2, soothing_width_eV=w)
+        g, o = SBEContext(cfg), Oracle(cfg)
+        st = o.initial_state(o.fermi_level(cfg.occupied_bands))
+        x = cases.perturbed_state(st, o.n, o.P, o.num_k, seed=5)
+        assert rel(g.rhs(x, 0.0), o.rhs(x, 0.0)) < 1e-12
+
+
+def test_replay_chain_full_pulse():
+    """full 8-cycle propagation of the 1-D chain: the GPU replays the oracle's
+    accepted dt sequence; j(t) and rho agree to 1e-10; the GPU's own adaptive
+    run takes the same step sequence."""
+    say("[gpu] replay chain1d")
+    cfg = cases.chain(sweeps=5)
+    g, o = SBEContext(cfg), Oracle(cfg)
+    mu = o.fermi_level(cfg.occupied_bands)
+    st0 = o.initial_state(mu)
+    so = st0.copy()
+    ro = o.integrate(so, cfg.output_count)
+    sg = st0.copy()
+    rg = g.integrate(sg, cfg.output_count, replay=ro["log"])
+    assert rg["steps"] == ro["steps"]
+    assert np.array_equal(rg["t"], ro["t"])
+    scale = np.abs(ro["j"]).max()
+    assert np.abs(rg["j"] - ro["j"]).max() < 1e-10 * scale
+    assert rel(sg, so) < 1e-10
+    # own step control
+    sa = st0.copy()
+    ra = g.integrate(sa, cfg.output_count)
+    say(f"  steps oracle {ro['steps']} gpu {ra['steps']}")
+    assert abs(ra["steps"] - ro["steps"]) <= 2
+    assert np.abs(ra["j"] - ro["j"]).max() < 1e-8 * scale
+
+
+def test_replay_square_short_pulse():
+    say("[gpu] replay square2d (15x15, 1 cycle)")
+    cfg = cases.small_square(eps_rel=1e-7)
+    g, o = SBEContext(cfg), Oracle(cfg)
+    mu = o.fermi_level(cfg.occupied_bands)
+    st0 = o.initial_state(mu)
+    so = st0.copy()
+    ro = o.integrate(so, cfg.output_count)
+    sg = st0.copy()
+    rg = g.integrate(sg, cfg.output_count, replay=ro["log"])
+    scale = np.abs(ro["j"]).max()
+    say(f"  steps {ro['steps']} rhs {ro['rhs_calls']} vs {rg['rhs_calls']}")
+    assert rg["rhs_calls"] == ro["rhs_calls"]
+    assert np.abs(rg["j"] - ro["j"]).max() < 1e-10 * scale
+    assert rel(sg, so) < 1e-10
