@@ -810,8 +810,19 @@ int oracle_observe_j(oracle_ctx *c, const double *state_d, double jout[3])
     }
     to_frac_k(c->lat, kOff, off);
     double js[3] = {0, 0, 0};
+    /* per-thread partial sums combined in thread order: deterministic for a
+     * fixed thread count (static schedule) */
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    double *part = calloc((size_t)nth * 3, sizeof(double));
 #pragma omp parallel
     {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
         cplx *H0 = malloc(sizeof(cplx) * nn), *D = malloc(sizeof(cplx) * 3 * nn), *dH = malloc(sizeof(cplx) * 3 * nn);
         cplx *T = malloc(sizeof(cplx) * nn);
         double loc[3] = {0, 0, 0};
@@ -837,13 +848,15 @@ int oracle_observe_j(oracle_ctx *c, const double *state_d, double jout[3])
                 loc[d] += creal(jk);
             }
         }
-#pragma omp critical
-        for (int d = 0; d < 3; d++) js[d] += loc[d];
+        for (int d = 0; d < 3; d++) part[3 * tid + d] = loc[d];
         free(H0);
         free(D);
         free(dH);
         free(T);
     }
+    for (int th = 0; th < nth; th++)
+        for (int d = 0; d < 3; d++) js[d] += part[3 * th + d];
+    free(part);
     for (int d = 0; d < 3; d++) jout[d] = js[d] / (double)c->nk;
     return 0;
 }

@@ -1,0 +1,63 @@
+"""The C-ABI library loads, exports every symbol include/sbewh_hip.h declares,
+and rejects invalid configurations with status codes before touching a GPU."""
+import ctypes as C
+import dataclasses
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sbewh_amd import _lib
+from sbewh_amd.ctypes_types import Inputs
+from tests import cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "sbewh_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sbe_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 19
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(_lib.EXPORTS)
+    assert L.sbe_version().decode().startswith("sbewh")
+
+
+def _create(cfg, mutate=None):
+    inp = Inputs(cfg)
+    if mutate:
+        mutate(inp)
+    h = C.c_void_p()
+    L = _lib.load()
+    rc = L.sbe_create(C.byref(inp.model), C.byref(inp.grid), C.byref(inp.params), inp.pulses, inp.num_pulses,
+                      C.byref(h))
+    return rc, L.sbe_last_error(None).decode(), h
+
+
+def test_invalid_configurations_are_rejected():
+    cfg = cases.chain()
+    rc, msg, h = _create(dataclasses.replace(cfg, Nk=(250, 1, 1)))
+    assert rc == -1 and "not a multiple of Nf" in msg and not h.value
+    rc, msg, _ = _create(cfg, lambda i: setattr(i.grid, "dim", 4))
+    assert rc == -1 and "dim" in msg
+    rc, msg, _ = _create(cfg, lambda i: setattr(i.params, "sweep_count", 99))
+    assert rc == -1
+    big = dataclasses.replace(cfg.model, H=np.zeros((9, 17, 17), complex), D=np.zeros((9, 3, 17, 17), complex))
+    rc, msg, _ = _create(dataclasses.replace(cfg, model=big))
+    assert rc == -1 and "num_wann" in msg
+    no_origin = dataclasses.replace(cfg.model, cells=cfg.model.cells + np.array([10, 0, 0], np.int32))
+    rc, msg, _ = _create(dataclasses.replace(cfg, model=no_origin))
+    assert rc == -1 and "R=(0,0,0)" in msg
+    rc, msg, _ = _create(cfg, lambda i: setattr(i, "num_pulses", 0))
+    assert rc == -1
+    L = _lib.load()
+    assert L.sbe_jacobi_batch(0, 5, 1, None, None, None) == -1
+    assert L.sbe_state_size(None) < 0
