@@ -112,13 +112,40 @@ int sbe_hk_mesh(sbe_ctx *ctx, const double A[3], const double E[3], double *out)
  * code/src/CuDiagonalizator.cu:45-51). */
 int sbe_jacobi_batch(int n, int sweeps, long count, const double *H, double *U, double *eig);
 
-/* Device-resident timing of `steps` Dormand-Prince step attempts at fixed
- * (t, dt) from the given host state (each attempt = 6 RHS evaluations,
- * stage algebra and error norm; the state is not advanced).  ms receives the
- * wall time of the timed attempts, stage_ms the summed duration of the fused
- * stage kernel measured with HIP events (may be NULL). */
-int sbe_bench_attempts(sbe_ctx *ctx, const double *state, double t, double dt, int warmup, int steps,
-                       double *ms, double *stage_ms, double *phase_ms);
+/* Layout chosen for the fused stage kernel by the built-in tuner (lanes per
+ * matrix row; 0 before the first RHS evaluation) and, if the tuner ran, the
+ * timed variants (up to 8) with their average launch time in ms.  No
+ * reference counterpart: the reference's GPU path has no tunable kernels. */
+int sbe_stage_tuning(const sbe_ctx *ctx, int *lanes_per_row, int *count, int *variants, double *ms);
+
+/* Device-resident timing of the adaptive propagation: `warmup`, then `steps`
+ * Dormand-Prince step attempts of the controller (accept / reject and
+ * step-size adaptation as in sbe_integrate, without truncation onto output
+ * times) from the host state at time t with initial step dt; the state
+ * advances on the device.  ms: wall time of the timed attempts; rhs: RHS
+ * evaluations among them; stage_ms / phase_ms (optional): average duration per
+ * RHS of the fused stage kernel and of the phase sum, from HIP events in a
+ * separate pass of up to 20 attempts. */
+int sbe_bench_steps(sbe_ctx *ctx, const double *state, double t, double dt, int warmup, int steps, double *ms,
+                    long *rhs, double *stage_ms, double *phase_ms);
+
+/* High-harmonic spectrum of a current j(t) (host buffers, nt samples of
+ * ncomp <= 3 components, uniform t): the reference's post-processing chain
+ * fadeout + zero padding + |FFT|^2 omega^(2 power) / N
+ * (scripts/util/hhg.py:9-25), masked to omega_min < omega < omega_max
+ * (scripts/util/common.py:187-201), j normalised by its global maximum and
+ * the per-component spectra summed (scripts/showExpectationValues.py:36-58).
+ * Writes up to `capacity` points; *count receives the number of points (if it
+ * exceeds capacity nothing is written and SBE_ERR_ARG is returned). */
+int sbe_hhg_spectrum(const double *t, const double *j, int nt, int ncomp, double d_omega, double omega_min,
+                     double omega_max, double fadeout_rel, int power, double *omega_out, double *S_out,
+                     int capacity, int *count);
+
+/* Per-harmonic yield Y_h = sum of S over omega in [h w0 - width w0, h w0 + width w0),
+ * h = 1..hmax (extractProfiles, scripts/ellipticityDependency.py:77-83);
+ * omega ascending. */
+int sbe_harmonic_yields(const double *omega, const double *S, int count, double omega0, double width, int hmax,
+                        double *yields);
 
 #ifdef __cplusplus
 }

@@ -57,9 +57,17 @@ typedef struct sbe_params {
     double gs_temp;           /* k_B T of the initial Fermi-Dirac state */
     double eps_abs;           /* RK45 absolute tolerance */
     double eps_rel;           /* RK45 relative tolerance */
-    int sweep_count;          /* Jacobi sweeps per diagonalisation */
-    int reserved;
+    int sweep_count;          /* Jacobi sweeps per diagonalisation (upper bound) */
+    int flags;                /* SBE_FLAG_* */
 } sbe_params;
+
+/* sbe_params.flags */
+enum {
+    /* start every Jacobi diagonalisation of the RHS from the identity (the
+     * reference's behaviour) instead of from the previous RHS call's
+     * eigenvectors at the same k-point */
+    SBE_FLAG_COLD_JACOBI = 1
+};
 
 enum sbe_pulse_type {
     SBE_PULSE_SIN2 = 0,       /* code/src/pulse/Sin2Pulse1D.cpp */

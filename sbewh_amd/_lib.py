@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsbewh_hip.so")
 EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_set_pulses", "sbe_state_size",
            "sbe_num_k", "sbe_get_nf", "sbe_time_window", "sbe_pulse_E", "sbe_fermi_level", "sbe_initial_state",
            "sbe_rhs", "sbe_rhs_device", "sbe_observe", "sbe_integrate", "sbe_hk_mesh", "sbe_jacobi_batch",
-           "sbe_bench_attempts")
+           "sbe_bench_steps", "sbe_stage_tuning", "sbe_hhg_spectrum", "sbe_harmonic_yields")
 
 _lib = None
 
@@ -58,7 +58,10 @@ def load():
     L.sbe_integrate.argtypes = [vp, dp, i, dp, dp, P(SbeSteplog), P(SbeSteplog), P(C.c_long)]
     L.sbe_hk_mesh.argtypes = [vp, dp, dp, dp]
     L.sbe_jacobi_batch.argtypes = [i, i, l, dp, dp, dp]
-    L.sbe_bench_attempts.argtypes = [vp, dp, d, d, i, i, dp, dp, dp]
+    L.sbe_bench_steps.argtypes = [vp, dp, d, d, i, i, dp, P(C.c_long), dp, dp]
+    L.sbe_stage_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), dp]
+    L.sbe_hhg_spectrum.argtypes = [dp, dp, i, i, d, d, d, d, i, dp, dp, i, P(C.c_int)]
+    L.sbe_harmonic_yields.argtypes = [dp, dp, i, d, d, i, dp]
     _lib = L
     return L
 

@@ -22,7 +22,8 @@ ARCH = "gfx950"
 # -packed-fp32-ops: keep the compiler from forming packed FP32 instructions
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{SRC}"]
-SOURCES = ["phase_kernels.hip", "stage_kernels.hip", "rk_kernels.hip", "observer_kernels.hip", "sbe_api.cpp"]
+SOURCES = ["phase_kernels.hip", "stage_kernels.hip", "rk_kernels.hip", "observer_kernels.hip", "sbe_api.cpp",
+           "spectrum.cpp"]
 HEADERS = ["device_common.hpp", "kernels.hpp"]
 
 

@@ -37,6 +37,46 @@ __device__ __forceinline__ cd shfl_xor(cd v, int mask)
     return cd{__shfl_xor(v.x, mask), __shfl_xor(v.y, mask)};
 }
 
+// Lane exchange lane ^ MASK for MASK < 16 with DPP moves (no LDS round trip):
+// quad_perm gives xor 1, 2, 3; row_half_mirror is xor 7 within 8 lanes and
+// row_mirror xor 15 within 16 lanes; other masks are compositions.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+
+template <int MASK>
+__device__ __forceinline__ int xor_dpp_i(int v)
+{
+    static_assert(MASK > 0 && MASK < 16, "DPP exchange covers masks 1..15");
+    if constexpr (MASK == 1) return dpp_i<0xB1>(v);        // quad_perm [1,0,3,2]
+    else if constexpr (MASK == 2) return dpp_i<0x4E>(v);   // quad_perm [2,3,0,1]
+    else if constexpr (MASK == 3) return dpp_i<0x1B>(v);   // quad_perm [3,2,1,0]
+    else if constexpr (MASK == 7) return dpp_i<0x141>(v);  // row_half_mirror
+    else if constexpr (MASK == 15) return dpp_i<0x140>(v); // row_mirror
+    else if constexpr (MASK < 8) return dpp_i<0x141>(xor_dpp_i<MASK ^ 7>(v));
+    else return dpp_i<0x140>(xor_dpp_i<MASK ^ 15>(v));
+}
+
+template <int MASK>
+__device__ __forceinline__ double xor_dpp(double v)
+{
+    long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+    lo = xor_dpp_i<MASK>(lo);
+    hi = xor_dpp_i<MASK>(hi);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// lane ^ MASK exchange of a complex value: DPP for MASK < 16, ds_bpermute otherwise
+template <int MASK>
+__device__ __forceinline__ cd xchg(cd v)
+{
+    if constexpr (MASK < 16) return cd{xor_dpp<MASK>(v.x), xor_dpp<MASK>(v.y)};
+    else return cd{__shfl_xor(v.x, MASK), __shfl_xor(v.y, MASK)};
+}
+
 // sum over aligned segments of `seg` lanes (seg a power of two <= 64)
 template <int seg>
 __device__ __forceinline__ double seg_sum(double v)

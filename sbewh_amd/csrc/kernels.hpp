@@ -54,8 +54,13 @@ struct StageArgs {
     double w;              // soothing width
     double E[3];
     int dephase;           // F2 > 0
+    double *ucache;        // [k][n*n] eigenvectors of the previous call (or null)
+    int warm;              // 1: start the Jacobi from ucache instead of the identity
 };
-hipError_t launch_rhs_stage(int N, const StageArgs &a, hipStream_t s);
+// N: padded band count (2, 4, 8, 16); LR: lanes per matrix row (layout variant)
+hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s);
+// LR values compiled for N (returns the count written to lr_out)
+int stage_variants(int N, int *lr_out, int cap);
 
 struct InitArgs {
     const double *mesh;    // [k][ops][n*n], H0 = op 0
@@ -70,17 +75,16 @@ struct InitArgs {
     double mu;
     double kT;
 };
-hipError_t launch_init_state(int N, const InitArgs &a, hipStream_t s);
+hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s);
 
 // ---- RK stage algebra (rk_kernels.hip) -----------------------------------
 // out = x + sum_{j<nt} c[j] * k[j]   over S complex entries
 hipError_t launch_combine(long S, const double *x, int nt, const double *const *k, const double *c,
                           double *out, hipStream_t s);
 // err_max = max_i |sum_j e[j] k[j]_i| / (ea + er (|x_i| + dt |k0_i|)), written as the bit
-// pattern of a non-negative double into *err_bits (must be zeroed before).
+// pattern of a non-negative double into *err_bits; partial holds >= 1024 doubles.
 hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
-                             double er, double dt, unsigned long long *err_bits, hipStream_t s);
-hipError_t launch_zero_u64(unsigned long long *p, hipStream_t s);
+                             double er, double dt, double *partial, unsigned long long *err_bits, hipStream_t s);
 
 // ---- observer (observer_kernels.hip) -------------------------------------
 struct ObsArgs {

@@ -40,6 +40,7 @@ __global__ void twiddle_kernel(PhaseGeom g, const double *a_src, int use_shift, 
 
 struct ContractArgs {
     const double *in;    // mode 0: [outer][W][inner][M]
+    int ti;              // i-tile height of the tiled kernel (<= TI)
     double *out;         // out mode 0: [outer][Nkd][inner][M]; 1: [k][M] via kmap
     const double *tw;    // [Nkd][W]
     int W, Nkd;
@@ -125,6 +126,66 @@ __global__ __launch_bounds__(256) void contract_kernel(ContractArgs a)
     }
 }
 
+// Tiled form of the same contraction: a block owns 256 consecutive columns
+// (o, in, m) and an i-tile of TI rows; each thread keeps its W inputs in
+// registers, the TI x W twiddle tile sits in LDS (broadcast reads), outputs
+// are written coalesced along m.
+constexpr int TI = 16;
+constexpr int WMAX = 16;
+
+__global__ __launch_bounds__(256) void contract_tiled_kernel(ContractArgs a)
+{
+    __shared__ cd tws[TI * WMAX];
+    const long cols_per_o = a.inner * a.M;
+    const long ncol = a.outer * cols_per_o;
+    const long col = (long)blockIdx.x * 256 + threadIdx.x;
+    const int i0 = blockIdx.y * a.ti;
+    const int W = a.W;
+    for (int e = threadIdx.x; e < a.ti * W; e += 256) {
+        int ii = e / W, r = e % W;
+        int i = i0 + ii;
+        cd t = mk(0, 0);
+        if (i < a.Nkd) {
+            const double *p = a.tw + 2 * ((long)i * W + r);
+            t = mk(p[0], p[1]);
+        }
+        tws[ii * WMAX + r] = t;
+    }
+    __syncthreads();
+    if (col >= ncol) return;
+    const long o = col / cols_per_o;
+    const long rem = col - o * cols_per_o;
+    const long in = rem / a.M;
+    const int m = (int)(rem - in * a.M);
+    cd x[WMAX];
+#pragma unroll
+    for (int r = 0; r < WMAX; r++) {
+        if (r < W) {
+            long src = (o * W + r) * a.inner + in;
+            if (a.in_mode == 0) {
+                const double *p = a.in + 2 * (src * a.M + m);
+                x[r] = mk(p[0], p[1]);
+            } else {
+                x[r] = model_value(a, src, m);
+            }
+        } else {
+            x[r] = mk(0, 0);
+        }
+    }
+    const int iend = min(a.ti, a.Nkd - i0);
+    for (int ii = 0; ii < iend; ii++) {
+        cd acc = mk(0, 0);
+#pragma unroll
+        for (int r = 0; r < WMAX; r++)
+            if (r < W) acc = cfma(tws[ii * WMAX + r], x[r], acc);
+        long dst = (o * a.Nkd + (i0 + ii)) * a.inner + in;
+        if (a.out_mode == 1) dst = a.kmap[dst];
+        double *p = a.out + 2 * (dst * a.M + m);
+        p[0] = acc.x;
+        p[1] = acc.y;
+    }
+}
+
 hipError_t launch_twiddles(const PhaseGeom &g, const double *a_src, int use_shift, double *tw, hipStream_t s)
 {
     int tot = g.Nk[0] * g.W[0] + g.Nk[1] * g.W[1] + g.Nk[2] * g.W[2];
@@ -171,8 +232,20 @@ hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, 
         bool last = (d == 0);
         a.out_mode = last ? 1 : 0;
         a.out = last ? mesh : bufs[nb];
-        long total = outer * ((g.Nk[d] + IB - 1) / IB) * inner * a.M;
-        hipLaunchKernelGGL(contract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+        if (a.W <= WMAX) {
+            long ncol = outer * inner * a.M;
+            long cb = (ncol + 255) / 256;
+            // smallest i-tile that still gives >= 2048 blocks (fills 256 CUs),
+            // larger tiles reuse the inputs held in registers more often
+            int ti = TI;
+            while (ti > 1 && cb * ((g.Nk[d] + ti - 1) / ti) < 2048) ti >>= 1;
+            a.ti = ti;
+            dim3 grid((unsigned)cb, (unsigned)((g.Nk[d] + ti - 1) / ti));
+            hipLaunchKernelGGL(contract_tiled_kernel, grid, dim3(256), 0, s, a);
+        } else {
+            long total = outer * ((g.Nk[d] + IB - 1) / IB) * inner * a.M;
+            hipLaunchKernelGGL(contract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         src = a.out;

@@ -72,15 +72,18 @@ struct ErrArgs {
     double e[6];
     double ea, er, dt;
     long S;
+    double *partial;      // [ERR_BLOCKS]
     unsigned long long *err_bits;
 };
 
+constexpr int ERR_BLOCKS = 1024;
+
+// pass 1: grid-stride max over the state, one partial per block
 __global__ __launch_bounds__(256) void error_kernel(ErrArgs a)
 {
     __shared__ double red[4];
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0;
-    if (i < a.S) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.S; i += (long)gridDim.x * blockDim.x) {
         double re = 0, im = 0;
 #pragma unroll
         for (int j = 0; j < 6; j++) {
@@ -90,20 +93,33 @@ __global__ __launch_bounds__(256) void error_kernel(ErrArgs a)
         }
         double2 xv = reinterpret_cast<const double2 *>(a.x)[i];
         double2 d1 = reinterpret_cast<const double2 *>(a.k[0])[i];
-        v = hypot(re, im) / (a.ea + a.er * (hypot(xv.x, xv.y) + a.dt * hypot(d1.x, d1.y)));
+        double num = sqrt(re * re + im * im);
+        double den = a.ea + a.er * (sqrt(xv.x * xv.x + xv.y * xv.y) + a.dt * sqrt(d1.x * d1.x + d1.y * d1.y));
+        v = fmax(v, num / den);
     }
+    v = seg_max<64>(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// pass 2: one block reduces the partials; writes the bit pattern of the max
+__global__ __launch_bounds__(256) void error_final_kernel(const double *partial, int nb, unsigned long long *err_bits)
+{
+    __shared__ double red[4];
+    double v = 0;
+    for (int i = threadIdx.x; i < nb; i += 256) v = fmax(v, partial[i]);
     v = seg_max<64>(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-        // non-negative doubles order like their bit patterns
-        atomicMax(a.err_bits, (unsigned long long)__double_as_longlong(m));
+        *err_bits = (unsigned long long)__double_as_longlong(m);
     }
 }
 
 hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
-                             double er, double dt, unsigned long long *err_bits, hipStream_t s)
+                             double er, double dt, double *partial, unsigned long long *err_bits, hipStream_t s)
 {
     ErrArgs a{};
     a.x = x;
@@ -115,16 +131,15 @@ hipError_t launch_error_norm(long S, const double *x, const double *const *k, co
     a.er = er;
     a.dt = dt;
     a.S = S;
+    a.partial = partial;
     a.err_bits = err_bits;
-    hipLaunchKernelGGL(error_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
-
-hipError_t launch_zero_u64(unsigned long long *p, hipStream_t s)
-{
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, p);
+    long need = (S + 255) / 256;
+    int nb = (int)(need < ERR_BLOCKS ? need : ERR_BLOCKS);
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(error_kernel, dim3(nb), dim3(256), 0, s, a);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, partial, nb, err_bits);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -142,7 +143,10 @@ struct sbe_ctx {
     double *tw = nullptr, *y1 = nullptr, *y2 = nullptr;
     double *mesh = nullptr, *mesh_obs = nullptr;
     double *st[10] = {};  // x, k1..k7, xt, xn
+    double *ucache = nullptr;  // [k][n*n] eigenvectors of the last RHS (warm starts)
+    bool ucache_valid = false;
     unsigned long long *err_bits = nullptr;
+    double *err_partial = nullptr;
     unsigned long long *err_host = nullptr;  // pinned
     double *partial = nullptr;
     double *jbuf = nullptr;
@@ -165,6 +169,50 @@ struct sbe_ctx {
     }
 
     double *x() { return st[0]; }
+
+    // stage-kernel layout (lanes per row), chosen by tune_stage()
+    int lr = 0;
+    double tune_ms[8] = {};
+    int tune_lr[8] = {};
+    int tune_count = 0;
+
+    static int default_lr(int N) { return N <= 4 ? 1 : 2; }
+
+    // Time every compiled layout variant of the stage kernel on the current
+    // inputs (3 timed launches after one warm-up each) and keep the fastest.
+    // SBEWH_STAGE_LR in the environment forces a variant.
+    void tune_stage(const StageArgs &a, hipStream_t s)
+    {
+        int lrs[8];
+        int nv = stage_variants(N, lrs, 8);
+        if (const char *env = std::getenv("SBEWH_STAGE_LR")) {
+            int want = std::atoi(env);
+            for (int v = 0; v < nv; v++)
+                if (lrs[v] == want) lr = want;
+            if (lr) return;
+        }
+        if (nv == 1) {
+            lr = lrs[0];
+            return;
+        }
+        float best = 1e30f;
+        tune_count = nv;
+        for (int v = 0; v < nv; v++) {
+            HIPCHK(launch_rhs_stage(N, lrs[v], a, s));
+            HIPCHK(hipEventRecord(ev[0], s));
+            for (int it = 0; it < 3; it++) HIPCHK(launch_rhs_stage(N, lrs[v], a, s));
+            HIPCHK(hipEventRecord(ev[1], s));
+            HIPCHK(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            tune_lr[v] = lrs[v];
+            tune_ms[v] = ms / 3;
+            if (ms < best) {
+                best = ms;
+                lr = lrs[v];
+            }
+        }
+    }
 
     void E_at(double t, double E[3]) const
     {
@@ -211,7 +259,12 @@ struct sbe_ctx {
         a.w = prm.soothing_width;
         for (int d = 0; d < 3; d++) a.E[d] = E[d];
         a.dephase = prm.relaxation_F2 > 0;
-        HIPCHK(launch_rhs_stage(N, a, s));
+        a.ucache = ucache;
+        a.warm = 0;
+        if (!lr) tune_stage(a, s);
+        a.warm = (ucache_valid && !(prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
+        HIPCHK(launch_rhs_stage(N, lr, a, s));
+        if (a.dephase) ucache_valid = true;
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
         rhs_calls++;
     }
@@ -276,8 +329,7 @@ struct sbe_ctx {
         }
         const double *ek[6] = {k1, k3, k4, k5, k6, k7};
         double e[6] = {dt * DC1, dt * DC3, dt * DC4, dt * DC5, dt * DC6, dt * DC7};
-        HIPCHK(launch_zero_u64(err_bits, s));
-        HIPCHK(launch_error_norm(S, X, ek, e, prm.eps_abs, prm.eps_rel, dt, err_bits, s));
+        HIPCHK(launch_error_norm(S, X, ek, e, prm.eps_abs, prm.eps_rel, dt, err_partial, err_bits, s));
         HIPCHK(hipMemcpyAsync(err_host, err_bits, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         double v;
@@ -293,13 +345,14 @@ struct sbe_ctx {
 
     ~sbe_ctx()
     {
-        double *ptrs[] = {Hb, Db, dHb, tw, y1, y2, mesh, mesh_obs, partial, jbuf};
+        double *ptrs[] = {Hb, Db, dHb, tw, y1, y2, mesh, mesh_obs, partial, jbuf, ucache};
         for (double *p : ptrs)
             if (p) (void)hipFree(p);
         for (double *p : st)
             if (p) (void)hipFree(p);
         if (kmap) (void)hipFree(kmap);
         if (err_bits) (void)hipFree(err_bits);
+        if (err_partial) (void)hipFree(err_partial);
         if (err_host) (void)hipHostFree(err_host);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -503,7 +556,9 @@ void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_params *
         p = dalloc<double>(2 * c->S);
         HIPCHK(hipMemset(p, 0, 16 * c->S));
     }
+    c->ucache = dalloc<double>(2 * c->nk * nn);
     c->err_bits = dalloc<unsigned long long>(1);
+    c->err_partial = dalloc<double>(1024);
     HIPCHK(hipHostMalloc((void **)&c->err_host, sizeof(unsigned long long), hipHostMallocDefault));
     c->partial = dalloc<double>(3 * c->nk);
 }
@@ -599,7 +654,7 @@ int sbe_fermi_level(sbe_ctx *ctx, int occupied_bands, int below_gap, double *mu)
         a.n = n;
         a.P = c.P;
         a.sweeps = c.prm.sweep_count;
-        HIPCHK(launch_init_state(c.N, a, c.stream));
+        HIPCHK(launch_init_state(c.N, c.lr ? c.lr : sbe_ctx::default_lr(c.N), a, c.stream));
         std::vector<double> ev((size_t)n * c.nk);
         HIPCHK(hipMemcpyAsync(ev.data(), eig, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
@@ -647,7 +702,7 @@ static void initial_state_dev(sbe_ctx &c, double mu, double *dst)
     a.sweeps = c.prm.sweep_count;
     a.mu = mu;
     a.kT = c.prm.gs_temp;
-    HIPCHK(launch_init_state(c.N, a, c.stream));
+    HIPCHK(launch_init_state(c.N, c.lr ? c.lr : sbe_ctx::default_lr(c.N), a, c.stream));
 }
 
 int sbe_initial_state(sbe_ctx *ctx, double mu, double *state)
@@ -838,7 +893,7 @@ int sbe_jacobi_batch(int n, int sweeps, long count, const double *H, double *U, 
         a.P = 8 * ((n * n + 7) / 8);
         a.sweeps = sweeps;
         int N = n <= 2 ? 2 : n <= 4 ? 4 : n <= 8 ? 8 : 16;
-        HIPCHK(launch_init_state(N, a, nullptr));
+        HIPCHK(launch_init_state(N, sbe_ctx::default_lr(N), a, nullptr));
         HIPCHK(hipDeviceSynchronize());
         if (U) HIPCHK(hipMemcpy(U, dU, 16 * (size_t)count * n * n, hipMemcpyDeviceToHost));
         if (eig) HIPCHK(hipMemcpy(eig, de, 8 * (size_t)count * n, hipMemcpyDeviceToHost));
@@ -849,41 +904,70 @@ int sbe_jacobi_batch(int n, int sweeps, long count, const double *H, double *U, 
     return rc;
 }
 
-int sbe_bench_attempts(sbe_ctx *ctx, const double *state, double t, double dt, int warmup, int steps, double *ms,
-                       double *stage_ms, double *phase_ms)
+int sbe_stage_tuning(const sbe_ctx *ctx, int *lanes_per_row, int *count, int *variants, double *ms)
 {
-    if (!ctx || !state || steps < 1 || warmup < 0) return SBE_ERR_ARG;
+    if (!ctx || !lanes_per_row) return SBE_ERR_ARG;
+    *lanes_per_row = ctx->lr;
+    if (count) *count = ctx->tune_count;
+    for (int v = 0; v < ctx->tune_count; v++) {
+        if (variants) variants[v] = ctx->tune_lr[v];
+        if (ms) ms[v] = ctx->tune_ms[v];
+    }
+    return SBE_OK;
+}
+
+int sbe_bench_steps(sbe_ctx *ctx, const double *state, double t, double dt, int warmup, int steps, double *ms,
+                    long *rhs, double *stage_ms, double *phase_ms)
+{
+    if (!ctx || !state || steps < 1 || warmup < 0 || !(dt > 0)) return SBE_ERR_ARG;
     return guarded(ctx, [&] {
         sbe_ctx &c = *ctx;
         HIPCHK(hipMemcpyAsync(c.x(), state, 16 * c.S, hipMemcpyHostToDevice, c.stream));
         c.rhs_dev(c.x(), c.st[1], t, c.stream);
-        for (int i = 0; i < warmup; i++) c.attempt(t, dt);
+        // the controller of integrate_adaptive without output truncation
+        auto run = [&](int count) {
+            for (int i = 0; i < count; i++) {
+                double err = c.attempt(t, dt);
+                if (!(err <= 1.0)) {
+                    if (std::isnan(err)) fail(SBE_ERR_STEP, "error norm is NaN at t=%g", t);
+                    dt *= std::max(0.9 * std::pow(err, -1.0 / 3.0), 0.2);
+                    continue;
+                }
+                c.accept();
+                t += dt;
+                if (err < 0.5) dt *= 0.9 * std::pow(std::max(std::pow(5.0, -5.0), err), -1.0 / 5.0);
+            }
+        };
+        run(warmup);
         HIPCHK(hipStreamSynchronize(c.stream));
-        // pass 1: wall time of the attempts
+        long r0 = c.rhs_calls;
         HIPCHK(hipEventRecord(c.ev[2], c.stream));
-        for (int i = 0; i < steps; i++) c.attempt(t, dt);
+        run(steps);
         HIPCHK(hipEventRecord(c.ev[3], c.stream));
         HIPCHK(hipEventSynchronize(c.ev[3]));
         float tot = 0;
         HIPCHK(hipEventElapsedTime(&tot, c.ev[2], c.ev[3]));
         if (ms) *ms = tot;
-        // pass 2 (optional): per-kernel durations from events around each launch
+        if (rhs) *rhs = c.rhs_calls - r0;
+        // optional second pass: per-kernel durations from events around each launch
         if (stage_ms || phase_ms) {
             c.timing = true;
             c.evused = 0;
-            for (int i = 0; i < steps; i++) c.attempt(t, dt);
+            run(std::min(steps, 20));
             HIPCHK(hipStreamSynchronize(c.stream));
             c.timing = false;
             double sm = 0, pm = 0;
+            long nr = 0;
             for (size_t e = 0; e + 2 < c.evused; e += 3) {
                 float a = 0, b = 0;
                 HIPCHK(hipEventElapsedTime(&a, c.evpool[e], c.evpool[e + 1]));
                 HIPCHK(hipEventElapsedTime(&b, c.evpool[e + 1], c.evpool[e + 2]));
                 pm += a;
                 sm += b;
+                nr++;
             }
-            if (stage_ms) *stage_ms = sm;
-            if (phase_ms) *phase_ms = pm;
+            if (stage_ms) *stage_ms = nr ? sm / nr : 0;
+            if (phase_ms) *phase_ms = nr ? pm / nr : 0;
         }
     });
 }

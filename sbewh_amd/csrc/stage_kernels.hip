@@ -16,17 +16,22 @@
 // decomposition; the dephasing term depends only on the converged
 // eigen-decomposition (invariant under eigenvector phases and under
 // rotations within degenerate subspaces, where gamma -> 0).
+//
+// LR (lanes per row) trades registers per lane against lanes per k-point; the
+// host tunes it per problem (one wavefront holds KPW = 64/(N LR) k-points).
+#include <utility>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace sbe {
 
-template <int N>
+template <int N, int LR>
 struct Geo {
-    static constexpr int LR = (N <= 8) ? 1 : N / 8;  // lanes per row
-    static constexpr int C = N / LR;                 // matrix entries per lane
-    static constexpr int LK = N * LR;                // lanes per k-point
-    static constexpr int KPW = 64 / LK;              // k-points per wavefront
+    static_assert(N * LR <= 64 && LR <= N, "layout must fit one wavefront");
+    static constexpr int C = N / LR;     // matrix entries per lane
+    static constexpr int LK = N * LR;    // lanes per k-point
+    static constexpr int KPW = 64 / LK;  // k-points per wavefront
     static constexpr int NN = N * N;
 };
 
@@ -36,16 +41,16 @@ struct Lane {
     bool valid;
 };
 
-template <int N>
+template <int N, int LR>
 __device__ __forceinline__ Lane lane_id(long nk)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
     Lane L;
     int lane = threadIdx.x;
     L.kk = lane / G::LK;
     int li = lane % G::LK;
-    L.i = li / G::LR;
-    L.h = li % G::LR;
+    L.i = li / LR;
+    L.h = li % LR;
     L.k = (long)blockIdx.x * G::KPW + L.kk;
     L.valid = L.k < nk;
     return L;
@@ -56,10 +61,10 @@ __device__ __forceinline__ void wsync() { __syncthreads(); }
 
 // coalesced copy of the wave's KPW matrices (n x n at src + k*stride + off,
 // complex) into LDS natural layout [kk][N][N], zero padded.
-template <int N>
+template <int N, int LR>
 __device__ __forceinline__ void load_mats(cd *lds, const double *src, long stride, long off, int n, long nk)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
     for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
         int kk = e / G::NN, idx = e % G::NN;
         int a = idx / N, b = idx % N;
@@ -73,16 +78,15 @@ __device__ __forceinline__ void load_mats(cd *lds, const double *src, long strid
     }
 }
 
-// product out[c] = sum_l X[i][l] Y[l][j_c] with X, Y in LDS (natural layout);
-// op selects conj variants: XOP 0 X, 1 conj(X^T) (i.e. X[i][l] := conj(Xs[l][i]));
-// YOP 0 Y, 1 conj(Y^T).
-template <int N, int XOP, int YOP>
+// out[c] = sum_l X[i][l] Y[l][j_c] with X, Y in LDS (natural layout);
+// XOP 1 uses X[i][l] := conj(Xs[l][i]); YOP 1 uses Y[l][j] := conj(Ys[j][l]).
+template <int N, int LR, int XOP, int YOP>
 __device__ __forceinline__ void lds_product(const cd *X, const cd *Y, const Lane &L, cd *out)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
 #pragma unroll
     for (int c = 0; c < G::C; c++) out[c] = mk(0, 0);
-#pragma unroll 4
+#pragma unroll 2
     for (int l = 0; l < N; l++) {
         cd x = (XOP == 0) ? X[L.i * N + l] : cconj(X[l * N + L.i]);
 #pragma unroll
@@ -94,12 +98,20 @@ __device__ __forceinline__ void lds_product(const cd *X, const cd *Y, const Lane
     }
 }
 
-template <int N>
+template <int N, int LR>
 __device__ __forceinline__ void store_xrow(cd *lds, const Lane &L, const cd *v)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
 #pragma unroll
     for (int c = 0; c < G::C; c++) lds[L.i * N + (L.i ^ (L.h * G::C + c))] = v[c];
+}
+
+template <int N, int LR>
+__device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
+{
+    using G = Geo<N, LR>;
+#pragma unroll
+    for (int c = 0; c < G::C; c++) v[c] = lds[L.i * N + (L.i ^ (L.h * G::C + c))];
 }
 
 __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s)
@@ -116,19 +128,145 @@ __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s
     }
 }
 
-// Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (rows), eigenvalues
-// in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Rp/Rq[KPW*N], Fa[KPW*N].
-template <int N>
-__device__ void wave_jacobi(cd *A, cd *U, const Lane &L, int sweeps, double *Dg, cd *Rp, cd *Rq, int *Fa)
+// column update of one element a (column j) with its partner d (column j ^ r)
+__device__ __forceinline__ cd col_update(cd a, cd d, bool jp, cd cpp, cd cpq)
 {
-    using G = Geo<N>;
+    if (jp) {
+        // A[:,p] = (1 + Rqq) A[:,p] - Rqp A[:,q],  Rqq = conj(Rpp), Rqp = -conj(Rpq)
+        const cd cqq = cconj(cpp), cqp = mk(-cpq.x, cpq.y);
+        return csub(cadd(a, cmul(cqq, a)), cmul(cqp, d));
+    }
+    // A[:,q] = -Rpq A[:,p] + (1 + Rpp) A[:,q]
+    return csub(cadd(a, cmul(cpp, a)), cmul(cpq, d));
+}
+
+// One round r of the parallel Jacobi sweep: rotations on the pairs (p, p ^ r).
+template <int N, int LR, int R>
+__device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double thr, double *Dg, cd *Rp, cd *Rq,
+                                          int *Fa, int &anyrot)
+{
+    using G = Geo<N, LR>;
     constexpr int C = G::C;
+    constexpr int rc = R % C, rh = R / C;
+    constexpr int mrow = R * LR + rh;   // lane mask of the row partner (row i ^ R)
     const int base = L.kk * N;
+    // (a) rotation parameters, computed by the owner of element (p, p^R), p < p^R
+    if (L.h == rh && L.i < (L.i ^ R)) {
+        cd apq = A[rc];
+        double app = Dg[base + L.i], aqq = Dg[base + (L.i ^ R)];
+        double mod = hypot(apq.x, apq.y);
+        int act = L.valid && (mod > thr);
+        cd rpp = mk(0, 0), rpq = mk(0, 0);
+        if (act) {
+            double c1, s1, c2, s2;
+            cs_half(-apq.x, apq.y, c1, s1);
+            cs_half(2 * mod, app - aqq, c2, s2);
+            rpp = mk(-s1 * s2 - 1, -c1 * s2);
+            rpq = mk(-c1 * c2, -s1 * c2);
+        }
+        Rp[base + L.i] = rpp;
+        Rq[base + L.i] = rpq;
+        Fa[base + L.i] = act;
+    }
+    wsync();
+    const int prow = min(L.i, L.i ^ R);
+    const cd rpp = Rp[base + prow], rpq = Rq[base + prow];
+    const int arow = Fa[base + prow];
+    const bool isp = L.i < (L.i ^ R);
+    anyrot |= arow;
+    if (!__any(arow)) {
+        // no rotation in this round anywhere in the wavefront: A, U unchanged
+        wsync();
+        return;
+    }
+    // row-pair coefficients: new = ca * x_self + cb * x_partner
+    cd ca, cb;
+    if (isp) {
+        ca = mk(1 + rpp.x, rpp.y);
+        cb = rpq;
+    } else {
+        ca = mk(1 + rpp.x, -rpp.y);  // 1 + Rqq
+        cb = mk(-rpq.x, rpq.y);      // Rqp
+    }
 #pragma unroll
-    for (int c = 0; c < C; c++) U[c] = mk((L.h * C + c) == 0 ? 1.0 : 0.0, 0);
+    for (int c = 0; c < C; c++) {
+        const int c2 = c ^ rc;
+        if (c2 < c) continue;
+        // row update of the register pair (c, c2); the partner row holds them in (c2, c)
+        cd a0 = A[c], a1 = A[c2];
+        const cd b0 = xchg<mrow>(A[c2]), b1 = xchg<mrow>(A[c]);
+        const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
+        if (arow) {
+            a0 = cfma(cb, b0, cmul(ca, a0));
+            U[c] = cfma(cb, v0, cmul(ca, U[c]));
+            if (c2 != c) {
+                a1 = cfma(cb, b1, cmul(ca, a1));
+                U[c2] = cfma(cb, v1, cmul(ca, U[c2]));
+            }
+        }
+        if (c2 == c) a1 = a0;
+        // column update: partner of column j is column j ^ R, register c ^ rc
+        cd d0, d1;
+        if constexpr (rh == 0) {
+            d0 = a1;
+            d1 = a0;
+        } else {
+            d0 = xchg<rh>(a1);
+            d1 = xchg<rh>(a0);
+        }
+        const int g0 = L.h * C + c, g1 = L.h * C + c2;
+        const int j0 = L.i ^ g0, j1 = L.i ^ g1;
+        const int pc0 = min(j0, j0 ^ R);
+        const int ac0 = Fa[base + pc0];
+        const cd p0 = Rp[base + pc0], q0 = Rq[base + pc0];
+        cd r0 = ac0 ? col_update(a0, d0, j0 < (j0 ^ R), p0, q0) : a0;
+        if (ac0 && g0 == R) r0 = mk(0, 0);
+        A[c] = r0;
+        if (c2 != c) {
+            cd r1;
+            if constexpr (rh == 0) {
+                // same column pair as c
+                r1 = ac0 ? col_update(a1, d1, j1 < (j1 ^ R), p0, q0) : a1;
+                if (ac0 && g1 == R) r1 = mk(0, 0);
+            } else {
+                const int pc1 = min(j1, j1 ^ R);
+                const int ac1 = Fa[base + pc1];
+                r1 = ac1 ? col_update(a1, d1, j1 < (j1 ^ R), Rp[base + pc1], Rq[base + pc1]) : a1;
+                if (ac1 && g1 == R) r1 = mk(0, 0);
+            }
+            A[c2] = r1;
+        }
+    }
     if (L.h == 0) Dg[base + L.i] = A[0].x;
     wsync();
+}
+
+template <int N, int LR, int... Rm1>
+__device__ __forceinline__ void jac_sweep(std::integer_sequence<int, Rm1...>, cd *A, cd *U, const Lane &L, double thr,
+                                          double *Dg, cd *Rp, cd *Rq, int *Fa, int &anyrot)
+{
+    (jac_round<N, LR, Rm1 + 1>(A, U, L, thr, Dg, Rp, Rq, Fa, anyrot), ...);
+}
+
+// Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (rows), eigenvalues
+// in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Rp/Rq[KPW*N], Fa[KPW*N].
+template <int N, int LR>
+__device__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Rp, cd *Rq, int *Fa,
+                            bool initU = true)
+{
+    using G = Geo<N, LR>;
+    constexpr int C = G::C;
+    if (initU) {
+#pragma unroll
+        for (int c = 0; c < C; c++) U[c] = mk((L0.h * C + c) == 0 ? 1.0 : 0.0, 0);
+    }
+    if (L0.h == 0) Dg[L0.kk * N + L0.i] = A[0].x;
+    wsync();
     for (int sw = 0; sw < sweeps; sw++) {
+        // re-materialise the lane coordinates each sweep so that the per-round
+        // LDS addresses are not hoisted out of the sweep loop (register pressure)
+        Lane L = L0;
+        asm volatile("" : "+v"(L.i), "+v"(L.h), "+v"(L.kk));
         double s = 0;
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -138,110 +276,15 @@ __device__ void wave_jacobi(cd *A, cd *U, const Lane &L, int sweeps, double *Dg,
         s = seg_sum<G::LK>(s);
         const double thr = 1e-15 * (0.5 * s);
         int anyrot = 0;
-#pragma unroll
-        for (int r = 1; r < N; r++) {
-            const int rc = r % C, rh = r / C;
-            const int mrow = r * G::LR + rh;
-            // (a) rotation parameters, computed by the owner of element (p, p^r), p < p^r
-            if (L.h == rh && L.i < (L.i ^ r)) {
-                cd apq = A[rc];
-                double app = Dg[base + L.i], aqq = Dg[base + (L.i ^ r)];
-                double mod = hypot(apq.x, apq.y);
-                int act = L.valid && (mod > thr);
-                cd rpp = mk(0, 0), rpq = mk(0, 0);
-                if (act) {
-                    double c1, s1, c2, s2;
-                    cs_half(-apq.x, apq.y, c1, s1);
-                    cs_half(2 * mod, app - aqq, c2, s2);
-                    rpp = mk(-s1 * s2 - 1, -c1 * s2);
-                    rpq = mk(-c1 * c2, -s1 * c2);
-                }
-                Rp[base + L.i] = rpp;
-                Rq[base + L.i] = rpq;
-                Fa[base + L.i] = act;
-            }
-            wsync();
-            const int prow = min(L.i, L.i ^ r);
-            const cd rpp = Rp[base + prow], rpq = Rq[base + prow];
-            const int arow = Fa[base + prow];
-            const bool isp = L.i < (L.i ^ r);
-            anyrot |= arow;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int c2 = c ^ rc;
-                if (c2 < c) continue;
-                // row update of the register pair (c, c2)
-                cd a0 = A[c], a1 = A[c2], u0 = U[c], u1 = U[c2];
-                cd b0 = shfl_xor(A[c2], mrow), b1 = shfl_xor(A[c], mrow);
-                cd v0 = shfl_xor(U[c2], mrow), v1 = shfl_xor(U[c], mrow);
-                if (arow) {
-                    if (isp) {
-                        // new_p = (1 + Rpp) x_p + Rpq x_q
-                        a0 = cfma(rpq, b0, cadd(a0, cmul(rpp, a0)));
-                        u0 = cfma(rpq, v0, cadd(u0, cmul(rpp, u0)));
-                        if (c2 != c) {
-                            a1 = cfma(rpq, b1, cadd(a1, cmul(rpp, a1)));
-                            u1 = cfma(rpq, v1, cadd(u1, cmul(rpp, u1)));
-                        }
-                    } else {
-                        // new_q = Rqp x_p + (1 + Rqq) x_q, Rqp = -conj(Rpq), Rqq = conj(Rpp)
-                        const cd rqp = mk(-rpq.x, rpq.y), rqq = cconj(rpp);
-                        a0 = cfma(rqp, b0, cadd(a0, cmul(rqq, a0)));
-                        u0 = cfma(rqp, v0, cadd(u0, cmul(rqq, u0)));
-                        if (c2 != c) {
-                            a1 = cfma(rqp, b1, cadd(a1, cmul(rqq, a1)));
-                            u1 = cfma(rqp, v1, cadd(u1, cmul(rqq, u1)));
-                        }
-                    }
-                }
-                U[c] = u0;
-                if (c2 != c) U[c2] = u1;
-                if (c2 == c) a1 = a0;
-                // column update of the pair: partner column j ^ r holds register c ^ rc
-                cd d0, d1;
-                if (rh == 0) {
-                    d0 = a1;
-                    d1 = a0;
-                } else {
-                    d0 = shfl_xor(a1, rh);
-                    d1 = shfl_xor(a0, rh);
-                }
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    if (t == 1 && c2 == c) break;
-                    const int cc = t ? c2 : c;
-                    const cd a = t ? a1 : a0, d = t ? d1 : d0;
-                    const int g = L.h * C + cc;
-                    const int j = L.i ^ g;
-                    const int pc = min(j, j ^ r);
-                    const int acol = Fa[base + pc];
-                    cd res = a;
-                    if (acol) {
-                        const cd cpp = Rp[base + pc], cpq = Rq[base + pc];
-                        if (j < (j ^ r)) {
-                            // A[:,p] = (1 + Rqq) A[:,p] - Rqp A[:,q]
-                            const cd cqq = cconj(cpp), cqp = mk(-cpq.x, cpq.y);
-                            res = csub(cadd(a, cmul(cqq, a)), cmul(cqp, d));
-                        } else {
-                            // A[:,q] = -Rpq A[:,p] + (1 + Rpp) A[:,q]
-                            res = csub(cadd(a, cmul(cpp, a)), cmul(cpq, d));
-                        }
-                        if (g == r) res = mk(0, 0);
-                    }
-                    A[cc] = res;
-                }
-            }
-            if (L.h == 0) Dg[base + L.i] = A[0].x;
-            wsync();
-        }
+        jac_sweep<N, LR>(std::make_integer_sequence<int, N - 1>{}, A, U, L, thr, Dg, Rp, Rq, Fa, anyrot);
         if (!__any(anyrot)) break;
     }
 }
 
-template <int N>
+template <int N, int LR>
 __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
     constexpr int C = G::C;
     __shared__ cd S0[G::KPW * G::NN];
     __shared__ cd S1[G::KPW * G::NN];
@@ -249,7 +292,7 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
     __shared__ cd Rp[G::KPW * N], Rq[G::KPW * N];
     __shared__ int Fa[G::KPW * N];
 
-    const Lane L = lane_id<N>(a.nk);
+    const Lane L = lane_id<N, LR>(a.nk);
     const int n = a.n, nn = n * n;
     cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
 
@@ -259,41 +302,58 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
         a.dxdt[2 * threadIdx.x + 1] = 0.0;
     }
 
-    // rho -> S0, H_E -> S1
-    load_mats<N>(S0, a.x, a.P, 8, n, a.nk);
-    load_mats<N>(S1, a.mesh, 2L * nn, nn, n, a.nk);
-    wsync();
-
-    // coherent part: -i (H_E rho - rho H_E)
     cd dr[C];
-    {
-        cd t1[C], t2[C];
-        lds_product<N, 0, 0>(s1, s0, L, t1);
-        lds_product<N, 0, 0>(s0, s1, L, t2);
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            cd d = csub(t1[c], t2[c]);
-            dr[c] = mk(d.y, -d.x);
-        }
-    }
+    for (int c = 0; c < C; c++) dr[c] = mk(0, 0);
 
     if (a.dephase) {
-        wsync();
-        load_mats<N>(S1, a.mesh, 2L * nn, 0, n, a.nk);  // H_0 -> S1
-        wsync();
         cd A[C], U[C];
+        if (a.warm) {
+            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
+            load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk);
+            load_mats<N, LR>(S1, a.mesh, 2L * nn, 0, n, a.nk);  // H_0 -> S1
+            wsync();
+            load_xrow<N, LR>(s0, L, U);
+            // padded rows of U' are the identity
 #pragma unroll
-        for (int c = 0; c < C; c++) A[c] = s1[L.i * N + (L.i ^ (L.h * C + c))];
-        wave_jacobi<N>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
-        // U -> S1 (H_0 is dead)
-        store_xrow<N>(s1, L, U);
+            for (int c = 0; c < C; c++)
+                if (L.i >= n) U[c] = mk((L.h * C + c) == 0 ? 1.0 : 0.0, 0);
+            cd T[C];
+            lds_product<N, LR, 0, 0>(s0, s1, L, T);  // U' H0
+            wsync();
+            store_xrow<N, LR>(s0, L, T);
+            store_xrow<N, LR>(s1, L, U);
+            wsync();
+            lds_product<N, LR, 0, 1>(s0, s1, L, A);  // (U' H0) U'^dagger
+            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa, false);
+        } else {
+            load_mats<N, LR>(S1, a.mesh, 2L * nn, 0, n, a.nk);  // H_0 -> S1
+            wsync();
+            load_xrow<N, LR>(s1, L, A);
+            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
+        }
+        store_xrow<N, LR>(s1, L, U);  // U -> S1
+        load_mats<N, LR>(S0, a.x, a.P, 8, n, a.nk);  // rho -> S0
         wsync();
+        if (a.ucache) {
+            // keep the eigenvectors for the next call's warm start
+            const long k0 = (long)blockIdx.x * G::KPW;
+            for (int e = threadIdx.x; e < G::KPW * nn; e += 64) {
+                int kk = e / nn, p = e % nn;
+                long k = k0 + kk;
+                if (k >= a.nk) break;
+                cd v = S1[kk * G::NN + (p / n) * N + (p % n)];
+                double *o = a.ucache + 2 * (k * nn + p);
+                o[0] = v.x;
+                o[1] = v.y;
+            }
+        }
         cd T[C];
-        lds_product<N, 0, 0>(s1, s0, L, T);  // T = U rho
+        lds_product<N, LR, 0, 0>(s1, s0, L, T);  // T = U rho
         wsync();
-        store_xrow<N>(s0, L, T);
+        store_xrow<N, LR>(s0, L, T);
         wsync();
-        lds_product<N, 0, 1>(s0, s1, L, T);  // rhoH = T U^dagger
+        lds_product<N, LR, 0, 1>(s0, s1, L, T);  // rhoH = T U^dagger
         const double di = Dg[L.kk * N + L.i];
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -308,20 +368,32 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
             T[c] = cscale(T[c], -gam * a.F2);
         }
         wsync();
-        store_xrow<N>(s0, L, T);
+        store_xrow<N, LR>(s0, L, T);
         wsync();
-        lds_product<N, 0, 0>(s0, s1, L, T);  // S = rhoH' U
+        lds_product<N, LR, 0, 0>(s0, s1, L, T);  // S = rhoH' U
         wsync();
-        store_xrow<N>(s0, L, T);
+        store_xrow<N, LR>(s0, L, T);
         wsync();
-        lds_product<N, 1, 0>(s1, s0, L, T);  // U^dagger S
+        lds_product<N, LR, 1, 0>(s1, s0, L, dr);  // U^dagger S
+        wsync();
+    }
+    load_mats<N, LR>(S0, a.x, a.P, 8, n, a.nk);  // rho (cache resident when reloaded)
+    // coherent part: -i (H_E rho - rho H_E)
+    load_mats<N, LR>(S1, a.mesh, 2L * nn, nn, n, a.nk);  // H_E -> S1
+    wsync();
+    {
+        cd t1[C];
+        lds_product<N, LR, 0, 0>(s1, s0, L, t1);
 #pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = cadd(dr[c], T[c]);
+        for (int c = 0; c < C; c++) dr[c] = mk(dr[c].x + t1[c].y, dr[c].y - t1[c].x);
+        lds_product<N, LR, 0, 0>(s0, s1, L, t1);
+#pragma unroll
+        for (int c = 0; c < C; c++) dr[c] = mk(dr[c].x - t1[c].y, dr[c].y + t1[c].x);
     }
 
     // write d rho/dt (with zero pads) through LDS for coalescing
     wsync();
-    store_xrow<N>(s0, L, dr);
+    store_xrow<N, LR>(s0, L, dr);
     wsync();
     const long k0 = (long)blockIdx.x * G::KPW;
     const int P = a.P;
@@ -337,27 +409,26 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
     }
 }
 
-// initial Fermi-Dirac state rho = U^dagger f(eps) U, and/or eigenvalues
-// (code/src/PropagatorCPU.cpp:31-70, code/src/main/runSBE.cpp:130-155)
-template <int N>
+// initial Fermi-Dirac state rho = U^dagger f(eps) U, and/or eigenvalues and
+// eigenvectors (code/src/PropagatorCPU.cpp:31-70, code/src/main/runSBE.cpp:130-155)
+template <int N, int LR>
 __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
     constexpr int C = G::C;
     __shared__ cd S0[G::KPW * G::NN];
     __shared__ cd S1[G::KPW * G::NN];
     __shared__ double Dg[G::KPW * N];
     __shared__ cd Rp[G::KPW * N], Rq[G::KPW * N];
     __shared__ int Fa[G::KPW * N];
-    const Lane L = lane_id<N>(a.nk);
+    const Lane L = lane_id<N, LR>(a.nk);
     const int n = a.n, nn = n * n;
     cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
-    load_mats<N>(S1, a.mesh, (long)a.ops * nn, 0, n, a.nk);
+    load_mats<N, LR>(S1, a.mesh, (long)a.ops * nn, 0, n, a.nk);
     wsync();
     cd A[C], U[C];
-#pragma unroll
-    for (int c = 0; c < C; c++) A[c] = s1[L.i * N + (L.i ^ (L.h * C + c))];
-    wave_jacobi<N>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
+    load_xrow<N, LR>(s1, L, A);
+    wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
     if (a.eig && L.valid && L.h == 0 && L.i < n) a.eig[L.k * n + L.i] = Dg[L.kk * N + L.i];
     if (a.U && L.valid && L.i < n) {
 #pragma unroll
@@ -372,18 +443,18 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     }
     if (!a.state) return;
     // U -> S1, f(eps) U -> S0
-    store_xrow<N>(s1, L, U);
+    store_xrow<N, LR>(s1, L, U);
     const double f = 1.0 / (1.0 + exp((Dg[L.kk * N + L.i] - a.mu) / a.kT));
     cd FU[C];
 #pragma unroll
     for (int c = 0; c < C; c++) FU[c] = (L.i < n) ? cscale(U[c], f) : mk(0, 0);
-    store_xrow<N>(s0, L, FU);
+    store_xrow<N, LR>(s0, L, FU);
     wsync();
     // rho[a][b] = sum_i conj(U[i][a]) f_i U[i][b]
     cd R[C];
-    lds_product<N, 1, 0>(s1, s0, L, R);
+    lds_product<N, LR, 1, 0>(s1, s0, L, R);
     wsync();
-    store_xrow<N>(s0, L, R);
+    store_xrow<N, LR>(s0, L, R);
     wsync();
     if (blockIdx.x == 0 && threadIdx.x < 8) {
         a.state[2 * threadIdx.x] = 0;
@@ -403,44 +474,67 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     }
 }
 
-template <int N>
-static hipError_t launch_rhs_N(const StageArgs &a, hipStream_t s)
+template <int N, int LR>
+static hipError_t launch_rhs_NL(const StageArgs &a, hipStream_t s)
 {
-    using G = Geo<N>;
+    using G = Geo<N, LR>;
     long blocks = (a.nk + G::KPW - 1) / G::KPW;
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rhs_stage_kernel<N>, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((rhs_stage_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_rhs_stage(int N, const StageArgs &a, hipStream_t s)
+template <int N, int LR>
+static hipError_t launch_init_NL(const InitArgs &a, hipStream_t s)
 {
-    switch (N) {
-    case 2: return launch_rhs_N<2>(a, s);
-    case 4: return launch_rhs_N<4>(a, s);
-    case 8: return launch_rhs_N<8>(a, s);
-    case 16: return launch_rhs_N<16>(a, s);
+    using G = Geo<N, LR>;
+    long blocks = (a.nk + G::KPW - 1) / G::KPW;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((init_state_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// (N, LR) variants compiled for the tuner; index order is the tuning order
+static const int kVariants[][2] = {{2, 1}, {2, 2}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}, {8, 4}, {16, 2}, {16, 4}};
+
+int stage_variants(int N, int *lr_out, int cap)
+{
+    int k = 0;
+    for (auto &v : kVariants)
+        if (v[0] == N && k < cap) lr_out[k++] = v[1];
+    return k;
+}
+
+hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s)
+{
+    switch (N * 100 + LR) {
+    case 201: return launch_rhs_NL<2, 1>(a, s);
+    case 202: return launch_rhs_NL<2, 2>(a, s);
+    case 401: return launch_rhs_NL<4, 1>(a, s);
+    case 402: return launch_rhs_NL<4, 2>(a, s);
+    case 404: return launch_rhs_NL<4, 4>(a, s);
+    case 801: return launch_rhs_NL<8, 1>(a, s);
+    case 802: return launch_rhs_NL<8, 2>(a, s);
+    case 804: return launch_rhs_NL<8, 4>(a, s);
+    case 1602: return launch_rhs_NL<16, 2>(a, s);
+    case 1604: return launch_rhs_NL<16, 4>(a, s);
     }
     return hipErrorInvalidValue;
 }
 
-template <int N>
-static hipError_t launch_init_N(const InitArgs &a, hipStream_t s)
+hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s)
 {
-    using G = Geo<N>;
-    long blocks = (a.nk + G::KPW - 1) / G::KPW;
-    if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(init_state_kernel<N>, dim3((unsigned)blocks), dim3(64), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_init_state(int N, const InitArgs &a, hipStream_t s)
-{
-    switch (N) {
-    case 2: return launch_init_N<2>(a, s);
-    case 4: return launch_init_N<4>(a, s);
-    case 8: return launch_init_N<8>(a, s);
-    case 16: return launch_init_N<16>(a, s);
+    switch (N * 100 + LR) {
+    case 201: return launch_init_NL<2, 1>(a, s);
+    case 202: return launch_init_NL<2, 2>(a, s);
+    case 401: return launch_init_NL<4, 1>(a, s);
+    case 402: return launch_init_NL<4, 2>(a, s);
+    case 404: return launch_init_NL<4, 4>(a, s);
+    case 801: return launch_init_NL<8, 1>(a, s);
+    case 802: return launch_init_NL<8, 2>(a, s);
+    case 804: return launch_init_NL<8, 4>(a, s);
+    case 1602: return launch_init_NL<16, 2>(a, s);
+    case 1604: return launch_init_NL<16, 4>(a, s);
     }
     return hipErrorInvalidValue;
 }

@@ -28,7 +28,7 @@ class SbeGrid(C.Structure):
 class SbeParams(C.Structure):
     _fields_ = [("relaxation_F2", C.c_double), ("soothing_width", C.c_double),
                 ("gs_temp", C.c_double), ("eps_abs", C.c_double), ("eps_rel", C.c_double),
-                ("sweep_count", C.c_int), ("reserved", C.c_int)]
+                ("sweep_count", C.c_int), ("flags", C.c_int)]
 
 
 class SbeSubpulse(C.Structure):
@@ -109,6 +109,7 @@ class Inputs:
         self.params.eps_abs = cfg.eps_abs
         self.params.eps_rel = cfg.eps_rel
         self.params.sweep_count = cfg.sweeps
+        self.params.flags = 1 if getattr(cfg, "cold_jacobi", False) else 0
         self.pulses = pulse_array(cfg.pulses)
         self.num_pulses = len(cfg.pulses)
 

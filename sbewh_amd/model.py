@@ -119,6 +119,7 @@ class Config:
     occupied_bands: int = 1
     output_count: int = 1024
     k_global_shift: tuple = (0.0, 0.0, 0.0)
+    cold_jacobi: bool = False   # start each RHS's Jacobi from the identity
 
     @property
     def num_k(self) -> int:

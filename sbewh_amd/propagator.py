@@ -122,13 +122,23 @@ class SBEContext:
         check(load().sbe_hk_mesh(self.h, dptr(_v3(A)), dptr(_v3(E)), dptr(out.view(float).reshape(-1))), self.h)
         return out
 
-    def bench_attempts(self, state: np.ndarray, t: float, dt: float, warmup: int, steps: int,
-                       kernel_timing: bool = True) -> dict:
+    def bench_steps(self, state: np.ndarray, t: float, dt: float, warmup: int, steps: int,
+                    kernel_timing: bool = True) -> dict:
+        """time `steps` adaptive controller attempts (after `warmup`) advancing a
+        device copy of `state` from time t; see sbe_bench_steps."""
         ms, sms, pms = C.c_double(), C.c_double(), C.c_double()
-        check(load().sbe_bench_attempts(self.h, dptr(state.view(float)), t, dt, warmup, steps, C.byref(ms),
-                                        C.byref(sms) if kernel_timing else None,
-                                        C.byref(pms) if kernel_timing else None), self.h)
-        return {"ms": ms.value, "stage_ms": sms.value, "phase_ms": pms.value}
+        rhs = C.c_long()
+        check(load().sbe_bench_steps(self.h, dptr(state.view(float)), t, dt, warmup, steps, C.byref(ms),
+                                     C.byref(rhs), C.byref(sms) if kernel_timing else None,
+                                     C.byref(pms) if kernel_timing else None), self.h)
+        return {"ms": ms.value, "rhs": rhs.value, "stage_ms": sms.value, "phase_ms": pms.value}
+
+    def stage_tuning(self) -> dict:
+        lr, cnt = C.c_int(), C.c_int()
+        var = (C.c_int * 8)()
+        ms = (C.c_double * 8)()
+        check(load().sbe_stage_tuning(self.h, C.byref(lr), C.byref(cnt), var, ms), self.h)
+        return {"lanes_per_row": lr.value, "timed": {int(var[i]): float(ms[i]) for i in range(cnt.value)}}
 
     def rho(self, state: np.ndarray) -> np.ndarray:
         n = self.n

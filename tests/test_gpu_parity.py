@@ -75,18 +75,25 @@ def test_fermi_initial_rhs_observe(name):
     j_g = g.observe(x)
     j_o = o.observe_j(x)
     assert np.abs(j_g - j_o).max() < 1e-12 * max(np.abs(j_o).max(), 1e-30)
+    # second call: the Jacobi starts from the eigenvectors of the first (warm start)
+    x2 = x.copy()
+    x2[0] += 0.013
+    t2 = 0.29 * o.t_start
+    d_g2, d_o2 = g.rhs(x2, t2), o.rhs(x2, t2)
+    assert rel(d_g2, d_o2) < 1e-12, rel(d_g2, d_o2)
 
 
-def test_rhs_no_dephasing_and_step_mask():
+def test_rhs_no_dephasing_step_mask_cold_jacobi():
     say("[gpu] rhs variants")
     import dataclasses
     base = cases.small_square()
-    for T2, w in ((-1.0, 0.025), (10.0, -1.0)):
-        cfg = dataclasses.replace(base, T2_fs=T2, soothing_width_eV=w)
+    for T2, w, cold in ((-1.0, 0.025, False), (10.0, -1.0, False), (10.0, 0.025, True)):
+        cfg = dataclasses.replace(base, T2_fs=T2, soothing_width_eV=w, cold_jacobi=cold)
         g, o = SBEContext(cfg), Oracle(cfg)
         st = o.initial_state(o.fermi_level(cfg.occupied_bands))
         x = cases.perturbed_state(st, o.n, o.P, o.num_k, seed=5)
-        assert rel(g.rhs(x, 0.0), o.rhs(x, 0.0)) < 1e-12
+        for t in (0.0, 0.01 * o.t_start):
+            assert rel(g.rhs(x, t), o.rhs(x, t)) < 1e-12
 
 
 def test_replay_chain_full_pulse():
@@ -127,6 +134,8 @@ def test_replay_square_short_pulse():
     rg = g.integrate(sg, cfg.output_count, replay=ro["log"])
     scale = np.abs(ro["j"]).max()
     say(f"  steps {ro['steps']} rhs {ro['rhs_calls']} vs {rg['rhs_calls']}")
-    assert rg["rhs_calls"] == ro["rhs_calls"]
+    # a replay runs only the accepted attempts: 6 RHS each + one FSAL start per interval
+    assert rg["steps"] == ro["steps"]
+    assert rg["rhs_calls"] == 6 * ro["steps"] + cfg.output_count - 1
     assert np.abs(rg["j"] - ro["j"]).max() < 1e-10 * scale
     assert rel(sg, so) < 1e-10
