@@ -61,3 +61,30 @@ def test_invalid_configurations_are_rejected():
     L = _lib.load()
     assert L.sbe_jacobi_batch(0, 5, 1, None, None, None) == -1
     assert L.sbe_state_size(None) < 0
+
+
+def test_cpp_propagator_header_links_against_library(tmp_path):
+    """include/sbewh_propagator.hpp compiles and links against the built library
+    (the program is only built, not run: running needs a GPU)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "use_header.cpp"
+    src.write_text(
+        '#include "sbewh_propagator.hpp"\n'
+        "int main(int argc, char **) {\n"
+        "  if (argc < 5) return 0;\n"
+        "  sbe_model m{}; sbe_grid g{}; sbe_params p{}; std::vector<sbe_subpulse> s(1);\n"
+        "  sbewh::Context c(m, g, p, s); sbewh::PropagatorHIP prop(c);\n"
+        "  sbewh::CPUstate x = prop.calcInitialState(0.0), dx;\n"
+        "  prop(x, dx, 0.0); sbewh::ObserverHIP obs(c); obs(x, 0.0);\n"
+        "  return (int)sbewh::integrate_const(c, x, 3, obs);\n"
+        "}\n")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    out = tmp_path / "use_header"
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", libdir, "-lsbewh_hip", "-Wl,-rpath," + libdir, "-o", str(out)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
