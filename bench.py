@@ -7,13 +7,15 @@ pulse (3 um, ellipticity 0.3, 10 MV/cm), T2 = 10 fs, soothing width 25 meV,
 5 Jacobi sweeps, FP64.
 
 One step = one Dormand-Prince 5(4) step attempt over the whole device-resident
-state: 6 RHS evaluations (phase sum + fused commutator/Jacobi/dephasing stage
-kernel), the stage algebra and the error-norm reduction read back by the host
+state: 6 RHS evaluations (partial phase sum + the fused stage kernel, which
+forms the stage input, finishes the phase sum and applies commutator, Jacobi
+and dephasing) and the error-norm reduction read back by the host
 controller.  value = N_k * 6 * K / seconds (k-point RHS evaluations per
 second).  The CPU baseline is the C/OpenMP oracle running the same attempts on
 the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --scan [--plain-sample P]   (spectra/hour, batched vs plain)
 """
 import argparse
 import json
@@ -29,16 +31,20 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 
-def stage_flops_per_k(n: int, sweeps: int) -> float:
+def stage_flops_per_k(n: int, sweeps: int, w_last: int) -> float:
     """algorithmic FLOPs of the fused stage kernel per k-point RHS: commutator
-    16 n^3, dephasing transforms 32 n^3, Jacobi 48 S n^2 (n-1)."""
-    return 16 * n ** 3 + 32 * n ** 3 + 48 * sweeps * n * n * (n - 1)
+    16 n^3, dephasing transforms 32 n^3, Jacobi 48 S n^2 (n-1), and the last
+    phase-sum contraction it performs while loading H0(k) and H_E(k):
+    2 operators x W_L complex multiply-adds x n^2 entries x 8 FLOP."""
+    return 16 * n ** 3 + 32 * n ** 3 + 48 * sweeps * n * n * (n - 1) + 16 * w_last * n * n
 
 
 def stage_bytes_per_k(n: int) -> float:
-    """algorithmic HBM bytes of the stage kernel per k-point: reads H0, H_E,
-    rho_in, writes d rho/dt (4 complex n x n)."""
-    return 4 * 16 * n * n
+    """algorithmic HBM bytes of the stage kernel per k-point RHS, averaged over
+    the six RHS of a Dormand-Prince attempt: reads x and the 10/3 stage
+    derivatives it combines on load, writes d rho/dt, and once per attempt the
+    new state (1/6): 5.5 complex n x n matrices."""
+    return 5.5 * 16 * n * n
 
 
 def pmc_traffic(path: str):
@@ -71,6 +77,96 @@ def cpu_baseline(cfg, state, t, dt, budget_s: float):
                       f"{el:.1f} s"}
 
 
+def spectra_of(t, j, omega0):
+    """spectrum + per-harmonic yields of one run (the a13 chain, C++ host code)"""
+    from sbewh_amd.spectrum import hhg_spectrum, harmonic_yields
+    w, S = hhg_spectrum(t, j)
+    return harmonic_yields(w, S, omega0)
+
+
+def scan_bench(args):
+    """64-pulse ellipticity / intensity / CEP scan over the 2-D 8-band model
+    with k-grids 16 x {60^2, 90^2, 120^2, 150^2}: complete propagations
+    (1024 outputs each) plus spectrum and harmonic yields, batched (one
+    sbe_scan) versus the plain solver (one sbe_integrate per pulse, one after
+    the other)."""
+    import numpy as np
+    from sbewh_amd import units
+    from sbewh_amd.propagator import SBEContext
+    from sbewh_amd.scan import Scan, ellipticity_scan
+
+    cfgs = ellipticity_scan(count=args.scan_count)
+    omega0 = units.omega_from_lambda_nm(cfgs[0].pulses[0].lambda_nm)
+    sync()
+    t0 = time.perf_counter()
+    sc = Scan(cfgs)
+    r = sc.run()
+    yields = [spectra_of(r["t"][i], r["j"][i], omega0) for i in range(len(cfgs))]
+    sync()
+    batched_s = time.perf_counter() - t0
+    tuning = sc.tuning()
+    sc.close()
+    del sc
+    # plain solver: the same pulses one after the other (all of them, or a
+    # stratified sample of `--plain-sample` per grid size, extrapolated)
+    by_grid = {}
+    for i, c in enumerate(cfgs):
+        by_grid.setdefault(c.Nk[0], []).append(i)
+    plain_runs = {}
+    for g, ids in sorted(by_grid.items()):
+        take = ids if args.plain_sample <= 0 else ids[:args.plain_sample]
+        for i in take:
+            c = cfgs[i]
+            sync()
+            t1 = time.perf_counter()
+            ctx = SBEContext(c)
+            x = ctx.initial_state(ctx.fermi_level(c.occupied_bands))
+            out = ctx.integrate(x, c.output_count)
+            y = spectra_of(out["t"], out["j"], omega0)
+            sync()
+            plain_runs[i] = (time.perf_counter() - t1, out, y)
+            ctx.close()
+    plain_s = 0.0
+    for g, ids in by_grid.items():
+        ts = [plain_runs[i][0] for i in ids if i in plain_runs]
+        plain_s += np.mean(ts) * len(ids)
+    # batched == plain on the instances run both ways
+    agree = max(float(np.abs(r["j"][i] - plain_runs[i][1]["j"]).max() / np.abs(plain_runs[i][1]["j"]).max())
+                for i in plain_runs)
+    steps_equal = all(int(r["steps"][i]) == plain_runs[i][1]["steps"] for i in plain_runs)
+    n = len(cfgs)
+    krhs = sum(c.num_k * int(r["rhs_calls"][i]) for i, c in enumerate(cfgs))
+    out = {
+        "metric": METRIC,
+        "value": n / batched_s * 3600.0,
+        "unit": "spectra/hour",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 0,
+        "ms_per_step": batched_s / n * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ellipticity scan: {n} elliptic sin2 pulses (3 um, 8 cycles, eps U[0,1], "
+                               "E0 U[3,15] MV/cm, CEP U[0,2pi)) over the 2-D 8-band model, k-grids "
+                               "{60,90,120,150}^2, T2 10 fs, rtol 1e-8, 1024 outputs, spectrum + yields H1-H40",
+                   "total_k": int(sum(c.num_k for c in cfgs)), "outputs": cfgs[0].output_count},
+        "batched": {"seconds": batched_s, "spectra_per_hour": n / batched_s * 3600.0,
+                    "k_rhs_per_s": krhs / batched_s, "steps_total": int(r["steps"].sum()),
+                    "rhs_calls_total": int(r["rhs_calls"].sum()), "stage_tuning": tuning},
+        "plain": {"seconds": plain_s, "spectra_per_hour": n / plain_s * 3600.0,
+                  "instances_timed": len(plain_runs),
+                  "extrapolated": len(plain_runs) < n},
+        "batched_over_plain": plain_s / batched_s,
+        "batched_vs_plain_max_rel_j": agree,
+        "batched_vs_plain_steps_equal": steps_equal,
+        "mean_log10_yield_H1_H40": float(np.mean(np.log10(np.maximum(np.array(yields), 1e-300)))),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,11 +174,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scan", action="store_true",
+                    help="spectra/hour of the 64-pulse batched scan versus the plain solver")
+    ap.add_argument("--scan-count", type=int, default=64)
+    ap.add_argument("--plain-sample", type=int, default=2,
+                    help="plain runs per grid size (0: all); the rest is extrapolated per grid size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
               "described in DESIGN.md but not built.", file=sys.stderr)
         sys.exit(2)
+    if args.scan:
+        scan_bench(args)
+        return
 
     from sbewh_amd import model
     from sbewh_amd.propagator import SBEContext
@@ -102,7 +206,7 @@ def main():
     value = nk * r["rhs"] / secs
     stage_avg_s = r["stage_ms"] / 1e3
     phase_avg_s = r["phase_ms"] / 1e3
-    flops_launch = stage_flops_per_k(n, S) * nk
+    flops_launch = stage_flops_per_k(n, S, ctx.last_axis_cells) * nk
     achieved = flops_launch / stage_avg_s / 1e12
     traffic = pmc_traffic(os.path.join(ROOT, "profiles", "stage_kernel_square2d_pmc.json"))
 

@@ -147,6 +147,42 @@ int sbe_hhg_spectrum(const double *t, const double *j, int nt, int ncomp, double
 int sbe_harmonic_yields(const double *omega, const double *S, int count, double omega0, double width, int hmax,
                         double *yields);
 
+/* ---- batched scan: many pulses / grids / T2 over one model -------------
+ * Replaces the reference's multi-run loop, which propagates the pulses of a
+ * multiRunFile one after the other with one integrate_const each and one
+ * shared initial state (code/src/main/runSBE.cpp:205-216; GPU:
+ * code/src/main/CuRunSBE.cu:55-71; configurations read by
+ * code/src/io/PulseParameter.cpp:201-211).  Here all instances advance
+ * together on one stream, each with its own adaptive step size and its own
+ * integrate_const output grid; per instance the results are those of
+ * sbe_integrate on a context created with the same inputs. */
+typedef struct sbe_scan sbe_scan;
+
+/* grid: dim, Nf and k_global_shift shared by all instances (Nk is taken from
+ * each instance); params: shared except relaxation_F2 (per instance);
+ * instance i uses pulses[pulse_offset .. pulse_offset + num_pulses). */
+int sbe_scan_create(const sbe_model *model, const sbe_grid *grid, const sbe_params *params,
+                    const sbe_instance *instances, int count, const sbe_subpulse *pulses, int total_pulses,
+                    sbe_scan **out);
+void sbe_scan_destroy(sbe_scan *scan);
+const char *sbe_scan_last_error(const sbe_scan *scan);
+
+/* Propagate every instance from the Fermi-Dirac ground state of its grid
+ * (occupied_bands, as sbe_fermi_level; computed once per distinct grid) over
+ * its pulse window with output_count outputs.  t_out: [count][output_count],
+ * j_out: [count][output_count][3], steps: [count] accepted steps (or a
+ * negative status for an instance whose step control failed; the others
+ * finish), rhs_calls: [count] (optional), seconds: wall time (optional). */
+int sbe_scan_run(sbe_scan *scan, int occupied_bands, int output_count, double *t_out, double *j_out, long *steps,
+                 long *rhs_calls, double *seconds);
+
+/* state of instance `index` after sbe_scan_run (sbe_scan_state_size complex entries) */
+long sbe_scan_state_size(const sbe_scan *scan, int index);
+int sbe_scan_final_state(const sbe_scan *scan, int index, double *state);
+
+/* stage-kernel layout chosen by the tuner on the batched launch (as sbe_stage_tuning) */
+int sbe_scan_tuning(const sbe_scan *scan, int *lanes_per_row, int *count, int *variants, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

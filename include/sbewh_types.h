@@ -94,6 +94,15 @@ typedef struct sbe_subpulse {
     double pol[3];
 } sbe_subpulse;
 
+/* One instance of a batched scan (sbe_scan_create): its own k-grid, pulse
+ * (a slice of the scan's sub-pulse array) and dephasing rate. */
+typedef struct sbe_instance {
+    int Nk[3];
+    int pulse_offset;
+    int num_pulses;
+    double relaxation_F2;     /* 1/T2; <= 0 disables dephasing */
+} sbe_instance;
+
 /* Record of the adaptive controller: one entry per step attempt. */
 typedef struct sbe_steplog {
     int capacity;             /* entries the arrays can hold */

@@ -7,7 +7,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .ctypes_types import SbeModel, SbeGrid, SbeParams, SbeSubpulse, SbeSteplog
+from .ctypes_types import SbeModel, SbeGrid, SbeParams, SbeSubpulse, SbeSteplog, SbeInstance
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libsbewh_hip.so")
@@ -16,7 +16,13 @@ LIB_PATH = os.path.join(HERE, "lib", "libsbewh_hip.so")
 EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_set_pulses", "sbe_state_size",
            "sbe_num_k", "sbe_get_nf", "sbe_time_window", "sbe_pulse_E", "sbe_fermi_level", "sbe_initial_state",
            "sbe_rhs", "sbe_rhs_device", "sbe_observe", "sbe_integrate", "sbe_hk_mesh", "sbe_jacobi_batch",
-           "sbe_bench_steps", "sbe_stage_tuning", "sbe_hhg_spectrum", "sbe_harmonic_yields")
+           "sbe_bench_steps", "sbe_stage_tuning", "sbe_hhg_spectrum", "sbe_harmonic_yields", "sbe_scan_create",
+           "sbe_scan_destroy", "sbe_scan_last_error", "sbe_scan_run", "sbe_scan_state_size", "sbe_scan_final_state",
+           "sbe_scan_tuning")
+# every symbol include/sbewh_surrogate.h declares
+SURROGATE_EXPORTS = ("sbe_mlp_create", "sbe_mlp_destroy", "sbe_mlp_last_error", "sbe_mlp_param_count",
+                     "sbe_mlp_get_params", "sbe_mlp_set_params", "sbe_mlp_forward", "sbe_mlp_gradient",
+                     "sbe_mlp_train")
 
 _lib = None
 
@@ -62,6 +68,28 @@ def load():
     L.sbe_stage_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), dp]
     L.sbe_hhg_spectrum.argtypes = [dp, dp, i, i, d, d, d, d, i, dp, dp, i, P(C.c_int)]
     L.sbe_harmonic_yields.argtypes = [dp, dp, i, d, d, i, dp]
+    L.sbe_scan_create.argtypes = [P(SbeModel), P(SbeGrid), P(SbeParams), P(SbeInstance), i, P(SbeSubpulse), i,
+                                  P(vp)]
+    L.sbe_scan_destroy.argtypes = [vp]
+    L.sbe_scan_last_error.restype = C.c_char_p
+    L.sbe_scan_last_error.argtypes = [vp]
+    L.sbe_scan_run.argtypes = [vp, i, i, dp, dp, P(C.c_long), P(C.c_long), dp]
+    L.sbe_scan_state_size.restype = l
+    L.sbe_scan_state_size.argtypes = [vp, i]
+    L.sbe_scan_final_state.argtypes = [vp, i, dp]
+    L.sbe_scan_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), dp]
+    fp = P(C.c_float)
+    L.sbe_mlp_create.argtypes = [i, i, P(C.c_int), C.c_ulonglong, P(vp)]
+    L.sbe_mlp_destroy.argtypes = [vp]
+    L.sbe_mlp_last_error.restype = C.c_char_p
+    L.sbe_mlp_last_error.argtypes = [vp]
+    L.sbe_mlp_param_count.restype = l
+    L.sbe_mlp_param_count.argtypes = [vp]
+    L.sbe_mlp_get_params.argtypes = [vp, i, fp]
+    L.sbe_mlp_set_params.argtypes = [vp, i, fp]
+    L.sbe_mlp_forward.argtypes = [vp, i, fp, fp]
+    L.sbe_mlp_gradient.argtypes = [vp, i, fp, fp, fp, fp]
+    L.sbe_mlp_train.argtypes = [vp, i, fp, fp, i, i, C.c_float, C.c_ulonglong, fp, P(C.c_double)]
     _lib = L
     return L
 

@@ -30,6 +30,46 @@ __device__ __forceinline__ cd cfma_jc(cd a, cd b, cd acc)
 {
     return cd{fma(a.x, b.x, fma(a.y, b.y, acc.x)), fma(-a.x, b.y, fma(a.y, b.x, acc.y))};
 }
+// Last-axis contraction of a partial phase sum (see PhaseLast in kernels.hpp)
+// for k-point k, entry e of an M-wide operator row.
+template <class PL>
+__device__ __forceinline__ cd phase_last(const PL &p, long k, int M, int e)
+{
+    const int2 ph = reinterpret_cast<const int2 *>(p.kphase)[k];
+    const double2 *y = reinterpret_cast<const double2 *>(p.Y) + (long)ph.x * M + e;
+    const double2 *t = reinterpret_cast<const double2 *>(p.twL) + ph.y;
+    cd v = mk(0, 0);
+#pragma unroll 4
+    for (int q = 0; q < p.WL; q++) {
+        const double2 tq = t[q], yq = y[(long)q * M];
+        v = cfma(mk(tq.x, tq.y), mk(yq.x, yq.y), v);
+    }
+    return v;
+}
+
+// Entry j of a batched launch owns the logical blocks [b0[j], b0[j+1]):
+// the largest j with b0[j] <= gb (b0 ascending, b0[0] == 0).
+__device__ __forceinline__ int find_entry(const long *b0, int m, long gb)
+{
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b0[mid] <= gb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD
+// b % 8).  This bijection gives each XCD one contiguous range of logical
+// blocks, so neighbouring k-points (which share phase-sum rows) meet in the
+// same XCD's L2.
+__device__ __forceinline__ long xcd_block(long b, long nb)
+{
+    const long q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 __device__ __forceinline__ double cabs2(cd a) { return a.x * a.x + a.y * a.y; }
 
 __device__ __forceinline__ cd shfl_xor(cd v, int mask)

@@ -21,11 +21,42 @@ struct PhaseGeom {
     double kshift_au[3];
 };
 
+// Stage input rho_in = x + sum_{j<nt} c[j] k[j] over the state: the
+// Dormand-Prince stage combination, formed by the kernels that consume it.
+struct StageInput {
+    const double *x;
+    const double *k[6];
+    double c[6];
+    int nt;
+};
+
 // Twiddles tw_d[i*W_d + r] = exp(i 2 pi (i/Nk_d + off_d)(r + rmin_d)), off =
-// frac(kshift_au + A).  A is read from Re(a_src[0..dim-1]) if a_src != null,
-// otherwise A = 0; use_shift = 0 drops kshift_au.
-hipError_t launch_twiddles(const PhaseGeom &g, const double *a_src, int use_shift, double *tw,
-                           hipStream_t s);
+// frac(kshift_au + A).  A is Re of the first dim special slots of the stage
+// input *in (A = 0 if in == null); use_shift = 0 drops kshift_au.
+hipError_t launch_twiddles(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw, hipStream_t s);
+
+// One contraction stage of the separable phase sum (see phase_kernels.hip).
+struct ContractArgs {
+    const double *in;    // mode 0: [outer][W][inner][M]
+    int ti;              // i-tile height of the tiled kernel
+    double *out;         // out mode 0: [outer][Nkd][inner][M]; 1: [k][M] via kmap
+    const double *tw;    // [Nkd][W]
+    int W, Nkd;
+    long outer, inner;
+    int M, nn, in_mode, out_mode, dim;
+    long nbox;
+    const double *Hb, *Db, *dHb;
+    double E0, E1, E2;
+    const int *kmap;
+};
+// a stage with its launch shape: kind 0 tiled contraction (grid blocks_x x
+// blocks_y), 1 generic contraction, 2 operator merge of the model box (dim 1)
+struct ContractStage {
+    ContractArgs a;
+    int kind;
+    long blocks_x;
+    int blocks_y;
+};
 
 // Full separable phase sum into mesh[k][nops][n*n] (state k order).
 //   ops == 2 : {H0, HE = H0 + sum_d E_d D_d}
@@ -40,11 +71,53 @@ struct PhaseBuffers {
 hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
                             double E2, double *mesh, hipStream_t s);
 
+// Phase sum over every dimension but the last one L = dim - 1 (the dimension
+// along which consecutive k indices run):
+//   Y[rest][r_L][ops nn],  rest = grid index over the dimensions < L,
+// ops == 2: {H0, H_E}; ops == 1 + 2 dim: {H0, D_0.., dH0/dk_0..}.
+// The consuming kernel contracts r_L itself.  For dim == 1, Y is the
+// operator-merged model box [W_0][ops nn].
+hipError_t launch_phase_partial(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
+                                double E2, double *Y, hipStream_t s);
+// the stages of launch_phase_partial (at most 2), returns their count
+int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1, double E2,
+                       double *Y, ContractStage *st);
+// complex entries of Y and of the stage scratch it needs (y1)
+long phase_partial_size(const PhaseGeom &g, int ops);
+long phase_partial_scratch(const PhaseGeom &g, int ops);
+// twiddle offset (complex entries) of dimension d inside tw
+long twiddle_offset(const PhaseGeom &g, int d);
+
+// Last-axis contraction of a partial phase sum for one k-point:
+//   X(k)[e] = sum_r twL[kphase[k].y + r] Y[(kphase[k].x + r) M + op nn + e]
+// with kphase[k] = (rest(k) W_L, i_L(k) W_L).
+struct PhaseLast {
+    const double *Y;
+    const double *twL;     // [Nk_L][W_L]
+    const int *kphase;     // int2 per k
+    int WL;
+};
+
+// ---- batched forms: several instances (own grid, pulse, buffers) per launch
+struct TwiddleEntry {
+    PhaseGeom g;
+    StageInput in;
+    int has_a;
+    int use_shift;
+    double *tw;
+};
+// grid: ceil(max_count / 256) x m, max_count = max_j sum_d Nk_d W_d
+hipError_t launch_twiddles_batched(const TwiddleEntry *d_entries, int m, int max_count, hipStream_t s);
+// entry j owns blocks [block0[j], block0[j] + blocks_x * blocks_y); blocks = total
+hipError_t launch_contract_batched(const ContractStage *d_stages, const long *d_block0, int m, long blocks,
+                                   hipStream_t s);
+
 // ---- fused RHS stage (stage_kernels.hip) ---------------------------------
 struct StageArgs {
-    const double *x;       // stage input state (reference layout)
+    StageInput in;         // stage input state (reference layout), combined on load
+    double *xout;          // if set, the combined stage input is also stored here
     double *dxdt;          // output state derivative
-    const double *mesh;    // [k][2][n*n]: H0, HE
+    PhaseLast ph;          // {H0, H_E} partial phase sum (ops == 2)
     long nk;
     int n;
     int P;                 // padded matrix stride in the state
@@ -61,6 +134,11 @@ struct StageArgs {
 hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s);
 // LR values compiled for N (returns the count written to lr_out)
 int stage_variants(int N, int *lr_out, int cap);
+// k-points per block (one wavefront) of layout (N, LR)
+int stage_k_per_block(int N, int LR);
+// entry j owns the logical blocks [block0[j], block0[j+1]) (ceil(nk_j / k_per_block) each)
+hipError_t launch_rhs_stage_batched(int N, int LR, const StageArgs *d_args, const long *d_block0, int m, long blocks,
+                                    hipStream_t s);
 
 struct InitArgs {
     const double *mesh;    // [k][ops][n*n], H0 = op 0
@@ -78,25 +156,38 @@ struct InitArgs {
 hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s);
 
 // ---- RK stage algebra (rk_kernels.hip) -----------------------------------
-// out = x + sum_{j<nt} c[j] * k[j]   over S complex entries
-hipError_t launch_combine(long S, const double *x, int nt, const double *const *k, const double *c,
-                          double *out, hipStream_t s);
 // err_max = max_i |sum_j e[j] k[j]_i| / (ea + er (|x_i| + dt |k0_i|)), written as the bit
 // pattern of a non-negative double into *err_bits; partial holds >= 1024 doubles.
 hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
                              double er, double dt, double *partial, unsigned long long *err_bits, hipStream_t s);
 
+// per-instance error norms: entry j covers state S_j of its instance and
+// writes its norm to *err_out (device); partial has >= sum_j blocks_j doubles
+struct ErrEntry {
+    const double *x;
+    const double *k[6];
+    double e[6];
+    double ea, er, dt;
+    long S;
+    double *err_out;
+};
+constexpr int ERR_CHUNK = 2048;   // state entries per block of the batched norm
+hipError_t launch_error_norm_batched(const ErrEntry *d_entries, const long *d_block0, int m, long blocks,
+                                     double *partial, hipStream_t s);
+
 // ---- observer (observer_kernels.hip) -------------------------------------
 struct ObsArgs {
     const double *x;       // state
-    const double *mesh;    // [k][1+2dim][n*n]: H0, D_d, dH_d
+    PhaseLast ph;          // observer partial phase sum (ops == 1 + 2 dim)
     long nk;
     int n;
     int P;
     int dim;
-    double *partial;       // [nblocks][3]
+    double *partial;       // [nk][3]
     double *j_out;         // 3 doubles (normalised by nk)
 };
 hipError_t launch_observe_j(const ObsArgs &a, hipStream_t s);
+// entry j owns blocks [block0[j], block0[j] + nk_j) (one block per k-point)
+hipError_t launch_observe_j_batched(const ObsArgs *d_args, const long *d_block0, int m, long blocks, hipStream_t s);
 
 }  // namespace sbe

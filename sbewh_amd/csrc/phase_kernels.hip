@@ -4,20 +4,26 @@
 // (code/src/WhFourierTransform.cpp:208-315; GPU: code/src/CuWhFourierTransform.cu:96-173).
 // Here the k + A shift is folded into per-dimension twiddles
 //   tw_d[i][r] = exp(i 2 pi (i/Nk_d + off_d) (r + rmin_d)),  off = frac(kGlobalShift + A),
-// and the sum over the dense R box is contracted one axis at a time
-// (R_2 -> i_2, R_1 -> i_1, R_0 -> i_0), i.e. a chain of small complex GEMMs
-// [Nk_d x W_d] x [W_d x (rest)].  The last contraction scatters into the
-// state's k order through a precomputed grid->k map.  The operator merge
-// H_E(R) = H(R) + sum_d E_d r_d(R) (code/src/WhFourierTransform.cpp:63-66) happens
-// while the first contraction reads the model.
+// and the sum over the dense R box is contracted one axis at a time, i.e. a
+// chain of small complex GEMMs [Nk_d x W_d] x [W_d x (rest)].  The operator
+// merge H_E(R) = H(R) + sum_d E_d r_d(R) (code/src/WhFourierTransform.cpp:63-66)
+// happens while the first contraction reads the model.
+//
+// Two forms:
+//  * launch_phase_sum: every axis (R_2 -> i_2, R_1 -> i_1, R_0 -> i_0), the
+//    last contraction scattering into the state's k order through the
+//    grid -> k map (observer operators, initial state, Fermi level);
+//  * launch_phase_partial: axes 0 .. dim-2 only (R_0 -> i_0 first), leaving
+//    the axis along which consecutive k indices run to the stage kernel,
+//    which contracts it while loading H(k) (no H(k) mesh in HBM).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace sbe {
 
-__global__ void twiddle_kernel(PhaseGeom g, const double *a_src, int use_shift, double *tw)
+__device__ __forceinline__ void twiddle_body(const PhaseGeom &g, const StageInput &in, int has_a, int use_shift,
+                                             double *tw, int t)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     int n0 = g.Nk[0] * g.W[0], n1 = g.Nk[1] * g.W[1], n2 = g.Nk[2] * g.W[2];
     if (t >= n0 + n1 + n2) return;
     int d = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
@@ -25,7 +31,14 @@ __global__ void twiddle_kernel(PhaseGeom g, const double *a_src, int use_shift, 
     int i = local / g.W[d], r = local % g.W[d];
     double k_au[3];
     for (int e = 0; e < 3; e++) {
-        double A = (a_src && e < g.dim) ? a_src[2 * e] : 0.0;
+        double A = 0.0;
+        if (has_a && e < g.dim) {
+            // same operation order as the stage input combination
+            A = in.x[2 * e];
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+                if (j < in.nt) A += in.c[j] * in.k[j][2 * e];
+        }
         k_au[e] = (use_shift ? g.kshift_au[e] : 0.0) + A;
     }
     // toFracK: (k . a_d) / 2 pi  (code/src/WhFourierTransform.cpp:46-52)
@@ -38,19 +51,17 @@ __global__ void twiddle_kernel(PhaseGeom g, const double *a_src, int use_shift, 
     tw[2 * t + 1] = s;
 }
 
-struct ContractArgs {
-    const double *in;    // mode 0: [outer][W][inner][M]
-    int ti;              // i-tile height of the tiled kernel (<= TI)
-    double *out;         // out mode 0: [outer][Nkd][inner][M]; 1: [k][M] via kmap
-    const double *tw;    // [Nkd][W]
-    int W, Nkd;
-    long outer, inner;
-    int M, nn, in_mode, out_mode, dim;
-    long nbox;
-    const double *Hb, *Db, *dHb;
-    double E0, E1, E2;
-    const int *kmap;
-};
+__global__ void twiddle_kernel(PhaseGeom g, StageInput in, int has_a, int use_shift, double *tw)
+{
+    twiddle_body(g, in, has_a, use_shift, tw, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// one instance per blockIdx.y
+__global__ void twiddle_batched_kernel(const TwiddleEntry *e)
+{
+    const TwiddleEntry &q = e[blockIdx.y];
+    twiddle_body(q.g, q.in, q.has_a, q.use_shift, q.tw, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 constexpr int IB = 8;
 
@@ -80,10 +91,10 @@ __device__ __forceinline__ cd model_value(const ContractArgs &a, long b, int m)
     return mk(p[0], p[1]);
 }
 
-__global__ __launch_bounds__(256) void contract_kernel(ContractArgs a)
+__device__ __forceinline__ void contract_body(const ContractArgs &a, long bx)
 {
     const int nIB = (a.Nkd + IB - 1) / IB;
-    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tid = bx * 256 + threadIdx.x;
     long total = a.outer * nIB * a.inner * a.M;
     if (tid >= total) return;
     int m = (int)(tid % a.M);
@@ -133,13 +144,13 @@ __global__ __launch_bounds__(256) void contract_kernel(ContractArgs a)
 constexpr int TI = 16;
 constexpr int WMAX = 16;
 
-__global__ __launch_bounds__(256) void contract_tiled_kernel(ContractArgs a)
+__device__ __forceinline__ void contract_tiled_body(const ContractArgs &a, long bx, int by)
 {
     __shared__ cd tws[TI * WMAX];
     const long cols_per_o = a.inner * a.M;
     const long ncol = a.outer * cols_per_o;
-    const long col = (long)blockIdx.x * 256 + threadIdx.x;
-    const int i0 = blockIdx.y * a.ti;
+    const long col = bx * 256 + threadIdx.x;
+    const int i0 = by * a.ti;
     const int W = a.W;
     for (int e = threadIdx.x; e < a.ti * W; e += 256) {
         int ii = e / W, r = e % W;
@@ -186,12 +197,87 @@ __global__ __launch_bounds__(256) void contract_tiled_kernel(ContractArgs a)
     }
 }
 
-hipError_t launch_twiddles(const PhaseGeom &g, const double *a_src, int use_shift, double *tw, hipStream_t s)
+// operator-merged model box Y[b][m] = op_m(R_b) (dim == 1)
+__device__ __forceinline__ void merge_body(const ContractArgs &a, long bx)
+{
+    long t = bx * 256 + threadIdx.x;
+    if (t >= a.nbox * a.M) return;
+    cd v = model_value(a, t / a.M, (int)(t % a.M));
+    a.out[2 * t] = v.x;
+    a.out[2 * t + 1] = v.y;
+}
+
+__device__ __forceinline__ void stage_body(const ContractStage &st, long lb)
+{
+    if (st.kind == 0) contract_tiled_body(st.a, lb % st.blocks_x, (int)(lb / st.blocks_x));
+    else if (st.kind == 1) contract_body(st.a, lb);
+    else merge_body(st.a, lb);
+}
+
+__global__ __launch_bounds__(256) void contract_stage_kernel(ContractStage st)
+{
+    stage_body(st, (long)blockIdx.y * st.blocks_x + blockIdx.x);
+}
+
+// entry j owns the blocks [block0[j], block0[j+1])
+__global__ __launch_bounds__(256) void contract_batched_kernel(const ContractStage *st, const long *block0, int m)
+{
+    const long gb = blockIdx.x;
+    const int j = find_entry(block0, m, gb);
+    const ContractStage q = st[j];
+    stage_body(q, gb - block0[j]);
+}
+
+hipError_t launch_twiddles(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw, hipStream_t s)
 {
     int tot = g.Nk[0] * g.W[0] + g.Nk[1] * g.W[1] + g.Nk[2] * g.W[2];
-    hipLaunchKernelGGL(twiddle_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, g, a_src, use_shift, tw);
+    StageInput v{};
+    if (in) v = *in;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, g, v, in ? 1 : 0, use_shift, tw);
     return hipGetLastError();
 }
+
+long twiddle_offset(const PhaseGeom &g, int d)
+{
+    long o = 0;
+    for (int e = 0; e < d; e++) o += (long)g.Nk[e] * g.W[e];
+    return o;
+}
+
+// grid of one contraction stage: the tiled kernel when the R extent fits its
+// register tile, the generic one otherwise
+static ContractStage plan_contract(const ContractArgs &a0)
+{
+    ContractStage st{};
+    st.a = a0;
+    ContractArgs &a = st.a;
+    if (a.W <= WMAX) {
+        long ncol = a.outer * a.inner * a.M;
+        long cb = (ncol + 255) / 256;
+        // smallest i-tile that still gives >= 2048 blocks (fills 256 CUs),
+        // larger tiles reuse the inputs held in registers more often
+        int ti = TI;
+        while (ti > 1 && cb * ((a.Nkd + ti - 1) / ti) < 2048) ti >>= 1;
+        a.ti = ti;
+        st.kind = 0;
+        st.blocks_x = cb;
+        st.blocks_y = (a.Nkd + ti - 1) / ti;
+    } else {
+        long total = a.outer * ((a.Nkd + IB - 1) / IB) * a.inner * a.M;
+        st.kind = 1;
+        st.blocks_x = (total + 255) / 256;
+        st.blocks_y = 1;
+    }
+    return st;
+}
+
+static void launch_stage(const ContractStage &st, hipStream_t s)
+{
+    hipLaunchKernelGGL(contract_stage_kernel, dim3((unsigned)st.blocks_x, (unsigned)st.blocks_y), dim3(256), 0, s,
+                       st);
+}
+
+static void launch_contract(const ContractArgs &a, hipStream_t s) { launch_stage(plan_contract(a), s); }
 
 // stage d runs iff it changes the data: W_d > 1 or Nk_d > 1 (stage 0 always runs)
 static bool stage_runs(const PhaseGeom &g, int d) { return d == 0 || g.W[d] > 1 || g.Nk[d] > 1; }
@@ -232,20 +318,7 @@ hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, 
         bool last = (d == 0);
         a.out_mode = last ? 1 : 0;
         a.out = last ? mesh : bufs[nb];
-        if (a.W <= WMAX) {
-            long ncol = outer * inner * a.M;
-            long cb = (ncol + 255) / 256;
-            // smallest i-tile that still gives >= 2048 blocks (fills 256 CUs),
-            // larger tiles reuse the inputs held in registers more often
-            int ti = TI;
-            while (ti > 1 && cb * ((g.Nk[d] + ti - 1) / ti) < 2048) ti >>= 1;
-            a.ti = ti;
-            dim3 grid((unsigned)cb, (unsigned)((g.Nk[d] + ti - 1) / ti));
-            hipLaunchKernelGGL(contract_tiled_kernel, grid, dim3(256), 0, s, a);
-        } else {
-            long total = outer * ((g.Nk[d] + IB - 1) / IB) * inner * a.M;
-            hipLaunchKernelGGL(contract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
-        }
+        launch_contract(a, s);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         src = a.out;
@@ -253,6 +326,97 @@ hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, 
         first = false;
     }
     return hipSuccess;
+}
+
+long phase_partial_size(const PhaseGeom &g, int ops)
+{
+    const long M = (long)ops * g.n * g.n;
+    const int L = g.dim - 1;
+    long v = M;
+    for (int e = 0; e < 3; e++) v *= (e < L) ? g.Nk[e] : g.W[e];
+    return v;
+}
+
+long phase_partial_scratch(const PhaseGeom &g, int ops)
+{
+    if (g.dim < 3) return 1;
+    return (long)g.Nk[0] * g.W[1] * g.W[2] * ops * g.n * g.n;
+}
+
+int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1, double E2,
+                       double *Y, ContractStage *st)
+{
+    const int nn = g.n * g.n;
+    ContractArgs a{};
+    a.M = ops * nn;
+    a.nn = nn;
+    a.dim = g.dim;
+    a.nbox = (long)g.W[0] * g.W[1] * g.W[2];
+    a.Hb = b.Hb;
+    a.Db = b.Db;
+    a.dHb = b.dHb;
+    a.E0 = g.dim > 0 ? E0 : 0;
+    a.E1 = g.dim > 1 ? E1 : 0;
+    a.E2 = g.dim > 2 ? E2 : 0;
+    const int model_mode = ops == 2 ? 1 : 2;
+    a.in_mode = model_mode;
+    const int L = g.dim - 1;
+    if (L == 0) {
+        a.out = Y;
+        st[0] = ContractStage{};
+        st[0].a = a;
+        st[0].kind = 2;
+        st[0].blocks_x = (a.nbox * a.M + 255) / 256;
+        st[0].blocks_y = 1;
+        return 1;
+    }
+    const double *src = nullptr;
+    for (int d = 0; d < L; d++) {
+        long outer = 1, inner = 1;
+        for (int e = 0; e < d; e++) outer *= g.Nk[e];
+        for (int e = d + 1; e < 3; e++) inner *= g.W[e];
+        a.in = src;
+        a.in_mode = d == 0 ? model_mode : 0;
+        a.W = g.W[d];
+        a.Nkd = g.Nk[d];
+        a.outer = outer;
+        a.inner = inner;
+        a.tw = b.tw + 2 * twiddle_offset(g, d);
+        a.out_mode = 0;
+        a.out = (d == L - 1) ? Y : b.y1;
+        st[d] = plan_contract(a);
+        src = a.out;
+    }
+    return L;
+}
+
+hipError_t launch_phase_partial(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
+                                double E2, double *Y, hipStream_t s)
+{
+    ContractStage st[2];
+    int ns = phase_partial_plan(g, b, ops, E0, E1, E2, Y, st);
+    for (int i = 0; i < ns; i++) {
+        launch_stage(st[i], s);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_twiddles_batched(const TwiddleEntry *d_entries, int m, int max_count, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(twiddle_batched_kernel, dim3((unsigned)((max_count + 255) / 256), (unsigned)m), dim3(256), 0,
+                       s, d_entries);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_batched(const ContractStage *d_stages, const long *d_block0, int m, long blocks,
+                                   hipStream_t s)
+{
+    if (m <= 0 || blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(contract_batched_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_stages, d_block0, m);
+    return hipGetLastError();
 }
 
 }  // namespace sbe

@@ -42,7 +42,7 @@ struct Lane {
 };
 
 template <int N, int LR>
-__device__ __forceinline__ Lane lane_id(long nk)
+__device__ __forceinline__ Lane lane_id(long nk, long blk)
 {
     using G = Geo<N, LR>;
     Lane L;
@@ -51,7 +51,7 @@ __device__ __forceinline__ Lane lane_id(long nk)
     int li = lane % G::LK;
     L.i = li / LR;
     L.h = li % LR;
-    L.k = (long)blockIdx.x * G::KPW + L.kk;
+    L.k = blk * G::KPW + L.kk;
     L.valid = L.k < nk;
     return L;
 }
@@ -62,19 +62,86 @@ __device__ __forceinline__ void wsync() { __syncthreads(); }
 // coalesced copy of the wave's KPW matrices (n x n at src + k*stride + off,
 // complex) into LDS natural layout [kk][N][N], zero padded.
 template <int N, int LR>
-__device__ __forceinline__ void load_mats(cd *lds, const double *src, long stride, long off, int n, long nk)
+__device__ __forceinline__ void load_mats(cd *lds, const double *src, long stride, long off, int n, long nk, long blk)
 {
     using G = Geo<N, LR>;
     for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
         int kk = e / G::NN, idx = e % G::NN;
         int a = idx / N, b = idx % N;
-        long k = (long)blockIdx.x * G::KPW + kk;
+        long k = blk * G::KPW + kk;
         cd v = mk(0, 0);
         if (a < n && b < n && k < nk) {
             const double *p = src + 2 * (k * stride + off + a * n + b);
             v = mk(p[0], p[1]);
         }
         lds[e] = v;
+    }
+}
+
+// H(k) of operator op (0: H0, 1: H_E) for the wave's k-points into LDS
+// natural layout: the last phase-sum contraction
+//   H(k)[a][b] = sum_r tw_L[i_L(k)][r] Y[rest(k)][r][op][a][b]
+// (the k-points of a wave share rest(k), so the Y rows are L1/L2 hits).
+template <int N, int LR>
+__device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, long blk)
+{
+    using G = Geo<N, LR>;
+    const int n = a.n;
+    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
+        int kk = e / G::NN, idx = e % G::NN;
+        int r = idx / N, c = idx % N;
+        long k = blk * G::KPW + kk;
+        cd v = mk(0, 0);
+        if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, 2 * n * n, op * n * n + r * n + c);
+        lds[e] = v;
+    }
+}
+
+// stage input rho = x + sum_j c_j k_j (the Dormand-Prince stage combination,
+// left to right as in the reference's scale_sum) into LDS natural layout
+template <int N, int LR>
+__device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
+{
+    using G = Geo<N, LR>;
+    const int n = a.n;
+    const StageInput &in = a.in;
+    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
+        int kk = e / G::NN, idx = e % G::NN;
+        int r = idx / N, c = idx % N;
+        long k = blk * G::KPW + kk;
+        cd v = mk(0, 0);
+        if (r < n && c < n && k < a.nk) {
+            const long i = 8 + k * a.P + r * n + c;
+            const double2 xv = reinterpret_cast<const double2 *>(in.x)[i];
+            double re = xv.x, im = xv.y;
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+                if (j < in.nt) {
+                    const double2 kv = reinterpret_cast<const double2 *>(in.k[j])[i];
+                    re += in.c[j] * kv.x;
+                    im += in.c[j] * kv.y;
+                }
+            v = mk(re, im);
+        }
+        lds[e] = v;
+    }
+}
+
+// the wave's matrices from LDS natural layout into a state vector (reference
+// layout, zero pads), coalesced
+template <int N, int LR>
+__device__ __forceinline__ void store_state(double *dst, const cd *lds, int n, int P, long nk, long blk)
+{
+    using G = Geo<N, LR>;
+    const int nn = n * n;
+    const long k0 = blk * G::KPW;
+    for (int e = threadIdx.x; e < G::KPW * P; e += 64) {
+        int kk = e / P, p = e % P;
+        long k = k0 + kk;
+        if (k >= nk) break;
+        cd v = mk(0, 0);
+        if (p < nn) v = lds[kk * G::NN + (p / n) * N + (p % n)];
+        reinterpret_cast<double2 *>(dst)[8 + k * P + p] = make_double2(v.x, v.y);
     }
 }
 
@@ -281,8 +348,9 @@ __device__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg
     }
 }
 
+// the stage for the KPW k-points of logical block blk of one instance
 template <int N, int LR>
-__global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
+__device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
 {
     using G = Geo<N, LR>;
     constexpr int C = G::C;
@@ -292,14 +360,26 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
     __shared__ cd Rp[G::KPW * N], Rq[G::KPW * N];
     __shared__ int Fa[G::KPW * N];
 
-    const Lane L = lane_id<N, LR>(a.nk);
+    const Lane L = lane_id<N, LR>(a.nk, blk);
     const int n = a.n, nn = n * n;
     cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
 
-    if (blockIdx.x == 0 && threadIdx.x < 8) {
-        double v = threadIdx.x < a.dim ? -a.E[threadIdx.x] : 0.0;
-        a.dxdt[2 * threadIdx.x] = v;
-        a.dxdt[2 * threadIdx.x + 1] = 0.0;
+    if (blk == 0 && threadIdx.x < 8) {
+        const int t = threadIdx.x;
+        double v = t < a.dim ? -a.E[t] : 0.0;
+        a.dxdt[2 * t] = v;
+        a.dxdt[2 * t + 1] = 0.0;
+        if (a.xout) {
+            double re = a.in.x[2 * t], im = a.in.x[2 * t + 1];
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+                if (j < a.in.nt) {
+                    re += a.in.c[j] * a.in.k[j][2 * t];
+                    im += a.in.c[j] * a.in.k[j][2 * t + 1];
+                }
+            a.xout[2 * t] = re;
+            a.xout[2 * t + 1] = im;
+        }
     }
 
     cd dr[C];
@@ -310,8 +390,8 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
         cd A[C], U[C];
         if (a.warm) {
             // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
-            load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk);
-            load_mats<N, LR>(S1, a.mesh, 2L * nn, 0, n, a.nk);  // H_0 -> S1
+            load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
+            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
             wsync();
             load_xrow<N, LR>(s0, L, U);
             // padded rows of U' are the identity
@@ -327,17 +407,20 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
             lds_product<N, LR, 0, 1>(s0, s1, L, A);  // (U' H0) U'^dagger
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa, false);
         } else {
-            load_mats<N, LR>(S1, a.mesh, 2L * nn, 0, n, a.nk);  // H_0 -> S1
+            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
             wsync();
             load_xrow<N, LR>(s1, L, A);
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
         }
         store_xrow<N, LR>(s1, L, U);  // U -> S1
-        load_mats<N, LR>(S0, a.x, a.P, 8, n, a.nk);  // rho -> S0
+        load_rho<N, LR>(S0, a, blk);  // rho -> S0
         wsync();
+        cd Rho[C];
+        load_xrow<N, LR>(s0, L, Rho);  // kept for the coherent part
+        if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
         if (a.ucache) {
             // keep the eigenvectors for the next call's warm start
-            const long k0 = (long)blockIdx.x * G::KPW;
+            const long k0 = blk * G::KPW;
             for (int e = threadIdx.x; e < G::KPW * nn; e += 64) {
                 int kk = e / nn, p = e % nn;
                 long k = k0 + kk;
@@ -376,10 +459,16 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
         wsync();
         lds_product<N, LR, 1, 0>(s1, s0, L, dr);  // U^dagger S
         wsync();
+        store_xrow<N, LR>(s0, L, Rho);
+    } else {
+        load_rho<N, LR>(S0, a, blk);
+        if (a.xout) {
+            wsync();
+            store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
+        }
     }
-    load_mats<N, LR>(S0, a.x, a.P, 8, n, a.nk);  // rho (cache resident when reloaded)
     // coherent part: -i (H_E rho - rho H_E)
-    load_mats<N, LR>(S1, a.mesh, 2L * nn, nn, n, a.nk);  // H_E -> S1
+    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
     wsync();
     {
         cd t1[C];
@@ -395,18 +484,24 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
     wsync();
     store_xrow<N, LR>(s0, L, dr);
     wsync();
-    const long k0 = (long)blockIdx.x * G::KPW;
-    const int P = a.P;
-    for (int e = threadIdx.x; e < G::KPW * P; e += 64) {
-        int kk = e / P, p = e % P;
-        long k = k0 + kk;
-        if (k >= a.nk) break;
-        cd v = mk(0, 0);
-        if (p < nn) v = S0[kk * G::NN + (p / n) * N + (p % n)];
-        double *o = a.dxdt + 2 * (8 + k * P + p);
-        o[0] = v.x;
-        o[1] = v.y;
-    }
+    store_state<N, LR>(a.dxdt, S0, n, a.P, a.nk, blk);
+}
+
+template <int N, int LR>
+__global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
+{
+    rhs_stage_body<N, LR>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// Several instances in one launch: entry j owns the logical blocks
+// [block0[j], block0[j+1]); its arguments are read once into registers.
+template <int N, int LR>
+__global__ __launch_bounds__(64) void rhs_stage_batched_kernel(const StageArgs *args, const long *block0, int m)
+{
+    const long gb = xcd_block(blockIdx.x, gridDim.x);
+    const int j = find_entry(block0, m, gb);
+    const StageArgs a = args[j];
+    rhs_stage_body<N, LR>(a, gb - block0[j]);
 }
 
 // initial Fermi-Dirac state rho = U^dagger f(eps) U, and/or eigenvalues and
@@ -421,10 +516,11 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     __shared__ double Dg[G::KPW * N];
     __shared__ cd Rp[G::KPW * N], Rq[G::KPW * N];
     __shared__ int Fa[G::KPW * N];
-    const Lane L = lane_id<N, LR>(a.nk);
+    const long blk = blockIdx.x;
+    const Lane L = lane_id<N, LR>(a.nk, blk);
     const int n = a.n, nn = n * n;
     cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
-    load_mats<N, LR>(S1, a.mesh, (long)a.ops * nn, 0, n, a.nk);
+    load_mats<N, LR>(S1, a.mesh, (long)a.ops * nn, 0, n, a.nk, blk);
     wsync();
     cd A[C], U[C];
     load_xrow<N, LR>(s1, L, A);
@@ -485,6 +581,16 @@ static hipError_t launch_rhs_NL(const StageArgs &a, hipStream_t s)
 }
 
 template <int N, int LR>
+static hipError_t launch_rhs_batched_NL(const StageArgs *args, const long *block0, int m, long blocks, hipStream_t s)
+{
+    if (blocks <= 0 || m <= 0) return hipSuccess;
+    hipLaunchKernelGGL((rhs_stage_batched_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, args, block0, m);
+    return hipGetLastError();
+}
+
+int stage_k_per_block(int N, int LR) { return 64 / (N * LR); }
+
+template <int N, int LR>
 static hipError_t launch_init_NL(const InitArgs &a, hipStream_t s)
 {
     using G = Geo<N, LR>;
@@ -518,6 +624,24 @@ hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s)
     case 804: return launch_rhs_NL<8, 4>(a, s);
     case 1602: return launch_rhs_NL<16, 2>(a, s);
     case 1604: return launch_rhs_NL<16, 4>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs_stage_batched(int N, int LR, const StageArgs *args, const long *block0, int m, long blocks,
+                                    hipStream_t s)
+{
+    switch (N * 100 + LR) {
+    case 201: return launch_rhs_batched_NL<2, 1>(args, block0, m, blocks, s);
+    case 202: return launch_rhs_batched_NL<2, 2>(args, block0, m, blocks, s);
+    case 401: return launch_rhs_batched_NL<4, 1>(args, block0, m, blocks, s);
+    case 402: return launch_rhs_batched_NL<4, 2>(args, block0, m, blocks, s);
+    case 404: return launch_rhs_batched_NL<4, 4>(args, block0, m, blocks, s);
+    case 801: return launch_rhs_batched_NL<8, 1>(args, block0, m, blocks, s);
+    case 802: return launch_rhs_batched_NL<8, 2>(args, block0, m, blocks, s);
+    case 804: return launch_rhs_batched_NL<8, 4>(args, block0, m, blocks, s);
+    case 1602: return launch_rhs_batched_NL<16, 2>(args, block0, m, blocks, s);
+    case 1604: return launch_rhs_batched_NL<16, 4>(args, block0, m, blocks, s);
     }
     return hipErrorInvalidValue;
 }

@@ -38,6 +38,11 @@ class SbeSubpulse(C.Structure):
                 ("center", C.c_double), ("rising_cycles", C.c_double), ("pol", C.c_double * 3)]
 
 
+class SbeInstance(C.Structure):
+    _fields_ = [("Nk", C.c_int * 3), ("pulse_offset", C.c_int), ("num_pulses", C.c_int),
+                ("relaxation_F2", C.c_double)]
+
+
 class SbeSteplog(C.Structure):
     _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("t", C.POINTER(C.c_double)),
                 ("dt", C.POINTER(C.c_double)), ("err", C.POINTER(C.c_double)),

@@ -58,6 +58,13 @@ class SBEContext:
         check(load().sbe_set_pulses(self.h, arr, len(pulses)), self.h)
 
     @property
+    def last_axis_cells(self) -> int:
+        """extent W_L of the model's R box along the last grid axis (the axis
+        the stage kernel contracts while loading H(k))."""
+        r = np.asarray(self.cfg.model.cells)[:, self.cfg.dim - 1]
+        return int(r.max() - r.min() + 1)
+
+    @property
     def nf(self):
         a = (C.c_int * 3)()
         check(load().sbe_get_nf(self.h, a), self.h)

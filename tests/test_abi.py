@@ -15,8 +15,8 @@ from tests import cases
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "sbewh_hip.h")).read()
+def declared_symbols(header="sbewh_hip.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(sbe_[A-Za-z0-9_]+)\s*\(", txt)))
 
@@ -29,6 +29,10 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, s), s
     assert set(syms) == set(_lib.EXPORTS)
     assert L.sbe_version().decode().startswith("sbewh")
+    surr = declared_symbols("sbewh_surrogate.h")
+    assert set(surr) == set(_lib.SURROGATE_EXPORTS)
+    for s in surr:
+        assert hasattr(L, s), s
 
 
 def _create(cfg, mutate=None):
