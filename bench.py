@@ -16,6 +16,7 @@ the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --scan [--plain-sample P]   (spectra/hour, batched vs plain)
+    python bench.py --surrogate                 (spectra/hour, surrogate + fallback vs plain)
 """
 import argparse
 import json
@@ -48,12 +49,17 @@ def stage_bytes_per_k(n: int) -> float:
 
 
 def pmc_traffic(path: str):
+    """HBM bytes per stage-kernel launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (calibrated against a device copy); the committed summary holds
+    the figure and the commands that produced it"""
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def sync():
@@ -100,6 +106,7 @@ def scan_bench(args):
     sync()
     t0 = time.perf_counter()
     sc = Scan(cfgs)
+    print(f"[batched] {len(cfgs)} pulses", file=sys.stderr, flush=True)
     r = sc.run()
     yields = [spectra_of(r["t"][i], r["j"][i], omega0) for i in range(len(cfgs))]
     sync()
@@ -126,6 +133,8 @@ def scan_bench(args):
             sync()
             plain_runs[i] = (time.perf_counter() - t1, out, y)
             ctx.close()
+            print(f"[plain] pulse {i} on {g}^2 k: {plain_runs[i][0]:.2f} s, {out['steps']} steps",
+                  file=sys.stderr, flush=True)
     plain_s = 0.0
     for g, ids in by_grid.items():
         ts = [plain_runs[i][0] for i in ids if i in plain_runs]
@@ -167,6 +176,67 @@ def scan_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def surrogate_bench(args):
+    """Spectra/hour of the learned surrogate with solver fallback versus the
+    plain solver, on the surrogate's domain (2-D 8-band model, 15 x 15 k,
+    elliptic sin2 pulses with E0, ellipticity, CEP, wavelength, cycles, T2
+    drawn from the training box): the held-out queries of the committed data
+    set go through Surrogate.query (network, and the solver for the queries
+    outside the box or with ensemble disagreement); the plain solver is timed
+    on a sample of the same queries, one sbe_integrate each."""
+    import numpy as np
+    from sbewh_amd.propagator import SBEContext
+    from sbewh_amd.surrogate import DATA_DIR, Surrogate, config_for, log_yields, relative_log_error
+
+    z = np.load(os.path.join(DATA_DIR, "surrogate_square2d_15x15.npz"), allow_pickle=False)
+    P, Y = z["params"][:512], z["log_yields"][:512]
+    s = Surrogate.load(os.path.join(DATA_DIR, "surrogate_ensemble.npz"))
+    s.predict(P[:8])
+    # two queries outside the training box exercise the fallback
+    Q = np.concatenate([P, [[1.0, 0.3, 0.5, 2000.0, 5.0, 10.0], [0.8, 0.5, 1.0, 4400.0, 6.0, 10.0]]])
+    sync()
+    t0 = time.perf_counter()
+    out, used, info = s.query(Q)
+    sync()
+    t_query = time.perf_counter() - t0
+    err = relative_log_error(out[:512], Y)
+    # plain solver on a sample of the queries
+    plain = []
+    for i in range(args.plain_sample_surrogate):
+        cfg = config_for(P[i])
+        sync()
+        t1 = time.perf_counter()
+        ctx = SBEContext(cfg)
+        x = ctx.initial_state(ctx.fermi_level(cfg.occupied_bands))
+        r = ctx.integrate(x, cfg.output_count)
+        ly = log_yields(r["t"], r["j"], cfg.pulses[0].lambda_nm)
+        sync()
+        plain.append(time.perf_counter() - t1)
+        ctx.close()
+        print(f"[plain] query {i}: {plain[-1]:.2f} s, {r['steps']} steps, max |dlog10 Y| vs data "
+              f"{np.abs(ly - Y[i]).max():.2e}", file=sys.stderr, flush=True)
+    plain_per = float(np.mean(plain))
+    n = len(Q)
+    value = n / t_query * 3600.0
+    out_json = {
+        "metric": METRIC, "value": value, "unit": "spectra/hour", "n_gpus": 1, "steps": n, "warmup": 8,
+        "ms_per_step": t_query / n * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "surrogate domain: 2-D 8-band model, 15x15 k, elliptic sin2 pulses, E0 0.3-1.5 V/nm, "
+                               "ellipticity 0-1, CEP 0-2pi, 2-4 um, 6-12 cycles, T2 5-20 fs; 512 held-out queries + "
+                               "2 outside the training box; log10 yields H1-H40",
+                   "queries": n},
+        "surrogate": {"seconds": t_query, "fallback_queries": int(used.sum()),
+                      "seconds_network": info["seconds_network"], "seconds_solver": info["seconds_solver"],
+                      "median_rel_log10_yield_error": float(np.median(err)),
+                      "p90_rel_log10_yield_error": float(np.quantile(err, 0.9))},
+        "plain": {"seconds_per_spectrum": plain_per, "spectra_per_hour": 3600.0 / plain_per, "sampled": len(plain)},
+        "batched_solver_spectra_per_hour": float(len(z["params"]) / z["seconds"].sum() * 3600.0),
+        "surrogate_over_plain": value / (3600.0 / plain_per),
+    }
+    print(json.dumps(out_json), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,13 +249,31 @@ def main():
     ap.add_argument("--scan-count", type=int, default=64)
     ap.add_argument("--plain-sample", type=int, default=2,
                     help="plain runs per grid size (0: all); the rest is extrapolated per grid size")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="spectra/hour of the surrogate with solver fallback versus the plain solver")
+    ap.add_argument("--plain-sample-surrogate", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
               "described in DESIGN.md but not built.", file=sys.stderr)
         sys.exit(2)
     if args.scan:
         scan_bench(args)
+        return
+    if args.surrogate:
+        surrogate_bench(args)
         return
 
     from sbewh_amd import model
@@ -208,7 +296,7 @@ def main():
     phase_avg_s = r["phase_ms"] / 1e3
     flops_launch = stage_flops_per_k(n, S, ctx.last_axis_cells) * nk
     achieved = flops_launch / stage_avg_s / 1e12
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "stage_kernel_square2d_pmc.json"))
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "stage_kernel_square2d_pmc.md"))
 
     out = {
         "metric": METRIC,

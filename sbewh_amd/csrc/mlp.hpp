@@ -32,11 +32,13 @@ struct GemmArgs {
 hipError_t launch_gemm(const GemmArgs &g, int ta, int tb, int members, hipStream_t s);
 
 // X[e][b][:] = bf16(Xall[perm[e][(step % spe) * batch + b]]), T likewise (fp32);
-// step = *counter.  perm: [members][n_perm].
+// step = *counter.  perm: [members][n_perm]; member e's rows start at row e * cap.
 hipError_t launch_gather(const float *Xall, const float *Tall, int din, int dout, const int *perm, long n_perm,
-                         const int *counter, int spe, int batch, int members, bf16 *X, float *T, hipStream_t s);
-// dY = 2 (Y - T) / (batch dout) in bf16; loss[e][*counter] = mean (Y - T)^2 (if loss != null)
-hipError_t launch_loss_grad(const float *Y, const float *T, int batch, int dout, int members, bf16 *dY,
+                         const int *counter, int spe, int batch, int cap, int members, bf16 *X, float *T,
+                         hipStream_t s);
+// dY = 2 (Y - T) / (batch dout) in bf16; loss[e][*counter] = mean (Y - T)^2 (if loss != null);
+// member e's rows start at row e * cap
+hipError_t launch_loss_grad(const float *Y, const float *T, int batch, int cap, int dout, int members, bf16 *dY,
                             float *loss, int loss_stride, const int *counter, hipStream_t s);
 // db[e][n] = sum_b dZ[e][b][n]
 hipError_t launch_colsum(const bf16 *dZ, int batch, int n, long strideZ, int members, float *db, long strideDb,

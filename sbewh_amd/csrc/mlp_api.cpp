@@ -323,7 +323,7 @@ int sbe_mlp_forward(sbe_mlp *mlp, int count, const float *x, float *y)
                 for (int e = 0; e < n.E; e++)
                     for (int b = 0; b < rows; b++) perm[(size_t)e * n.cap + b] = s0 + b;
                 MCHK(hipMemcpyAsync(dperm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, n.stream));
-                MCHK(launch_gather(dx, dx, din, 0, dperm, n.cap, dctr, 1, rows, n.E, n.X, n.T, n.stream));
+                MCHK(launch_gather(dx, dx, din, 0, dperm, n.cap, dctr, 1, rows, n.cap, n.E, n.X, n.T, n.stream));
                 n.forward(rows);
                 MCHK(hipMemcpyAsync(yb.data(), n.Y, yb.size() * sizeof(float), hipMemcpyDeviceToHost, n.stream));
                 MCHK(hipStreamSynchronize(n.stream));
@@ -362,9 +362,9 @@ int sbe_mlp_gradient(sbe_mlp *mlp, int count, const float *x, const float *t, fl
             MCHK(hipMemcpyAsync(dx, x, (size_t)count * din * sizeof(float), hipMemcpyHostToDevice, n.stream));
             MCHK(hipMemcpyAsync(dt, t, (size_t)count * dout * sizeof(float), hipMemcpyHostToDevice, n.stream));
             MCHK(hipMemsetAsync(dctr, 0, sizeof(int), n.stream));
-            MCHK(launch_gather(dx, dt, din, dout, dperm, count, dctr, 1, count, n.E, n.X, n.T, n.stream));
+            MCHK(launch_gather(dx, dt, din, dout, dperm, count, dctr, 1, count, n.cap, n.E, n.X, n.T, n.stream));
             n.forward(count);
-            MCHK(launch_loss_grad(n.Y, n.T, count, dout, n.E, n.dY, dl, 1, dctr, n.stream));
+            MCHK(launch_loss_grad(n.Y, n.T, count, n.cap, dout, n.E, n.dY, dl, 1, dctr, n.stream));
             n.backward(count);
             MCHK(hipMemcpyAsync(loss, dl, n.E * sizeof(float), hipMemcpyDeviceToHost, n.stream));
             MCHK(hipMemcpyAsync(grad, n.g, (size_t)n.E * n.P * sizeof(float), hipMemcpyDeviceToHost, n.stream));
@@ -408,9 +408,9 @@ int sbe_mlp_train(sbe_mlp *mlp, int count, const float *x, const float *t, int b
             MCHK(hipStreamSynchronize(n.stream));
             // one training step of every member, captured once and replayed
             MCHK(hipStreamBeginCapture(n.stream, hipStreamCaptureModeThreadLocal));
-            MCHK(launch_gather(dx, dt, din, dout, dperm, count, n.counter, spe, batch, n.E, n.X, n.T, n.stream));
+            MCHK(launch_gather(dx, dt, din, dout, dperm, count, n.counter, spe, batch, n.cap, n.E, n.X, n.T, n.stream));
             n.forward(batch);
-            MCHK(launch_loss_grad(n.Y, n.T, batch, dout, n.E, n.dY, dl, (int)total, n.counter, n.stream));
+            MCHK(launch_loss_grad(n.Y, n.T, batch, n.cap, dout, n.E, n.dY, dl, (int)total, n.counter, n.stream));
             n.backward(batch);
             MCHK(launch_adam(n.p, n.pb, n.m, n.v, n.g, (long)n.E * n.P, lr, 0.9f, 0.999f, 1e-8f, total, n.counter,
                              n.stream));

@@ -86,19 +86,21 @@ hipError_t launch_gemm(const GemmArgs &g, int ta, int tb, int members, hipStream
 
 // grid (batch, members), block 64
 __global__ void gather_kernel(const float *Xall, const float *Tall, int din, int dout, const int *perm, long n_perm,
-                              const int *counter, int spe, int batch, bf16 *X, float *T)
+                              const int *counter, int spe, int batch, int cap, bf16 *X, float *T)
 {
     const int b = blockIdx.x, e = blockIdx.y;
     const long idx = perm[e * n_perm + (long)(*counter % spe) * batch + b];
-    for (int i = threadIdx.x; i < din; i += 64) X[((long)e * batch + b) * din + i] = (bf16)Xall[idx * din + i];
-    for (int i = threadIdx.x; i < dout; i += 64) T[((long)e * batch + b) * dout + i] = Tall[idx * dout + i];
+    const long row = (long)e * cap + b;
+    for (int i = threadIdx.x; i < din; i += 64) X[row * din + i] = (bf16)Xall[idx * din + i];
+    for (int i = threadIdx.x; i < dout; i += 64) T[row * dout + i] = Tall[idx * dout + i];
 }
 
 hipError_t launch_gather(const float *Xall, const float *Tall, int din, int dout, const int *perm, long n_perm,
-                         const int *counter, int spe, int batch, int members, bf16 *X, float *T, hipStream_t s)
+                         const int *counter, int spe, int batch, int cap, int members, bf16 *X, float *T,
+                         hipStream_t s)
 {
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)batch, (unsigned)members), dim3(64), 0, s, Xall, Tall, din, dout,
-                       perm, n_perm, counter, spe, batch, X, T);
+                       perm, n_perm, counter, spe, batch, cap, X, T);
     return hipGetLastError();
 }
 
@@ -115,11 +117,11 @@ __device__ __forceinline__ float block_sum256(float v)
 }
 
 // one block per member, fixed summation order
-__global__ __launch_bounds__(256) void loss_grad_kernel(const float *Y, const float *T, int batch, int dout, bf16 *dY,
-                                                        float *loss, int loss_stride, const int *counter)
+__global__ __launch_bounds__(256) void loss_grad_kernel(const float *Y, const float *T, int batch, int cap, int dout,
+                                                        bf16 *dY, float *loss, int loss_stride, const int *counter)
 {
     const int e = blockIdx.x;
-    const long cnt = (long)batch * dout, off = e * cnt;
+    const long cnt = (long)batch * dout, off = (long)e * cap * dout;
     const float sc = 2.0f / (float)cnt;
     float acc = 0.0f;
     for (long i = threadIdx.x; i < cnt; i += 256) {
@@ -131,10 +133,10 @@ __global__ __launch_bounds__(256) void loss_grad_kernel(const float *Y, const fl
     if (loss && threadIdx.x == 0) loss[(long)e * loss_stride + *counter] = acc / (float)cnt;
 }
 
-hipError_t launch_loss_grad(const float *Y, const float *T, int batch, int dout, int members, bf16 *dY,
+hipError_t launch_loss_grad(const float *Y, const float *T, int batch, int cap, int dout, int members, bf16 *dY,
                             float *loss, int loss_stride, const int *counter, hipStream_t s)
 {
-    hipLaunchKernelGGL(loss_grad_kernel, dim3((unsigned)members), dim3(256), 0, s, Y, T, batch, dout, dY, loss,
+    hipLaunchKernelGGL(loss_grad_kernel, dim3((unsigned)members), dim3(256), 0, s, Y, T, batch, cap, dout, dY, loss,
                        loss_stride, counter);
     return hipGetLastError();
 }

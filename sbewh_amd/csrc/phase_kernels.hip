@@ -47,8 +47,7 @@ __device__ __forceinline__ void twiddle_body(const PhaseGeom &g, const StageInpu
     double ph = ((double)i / g.Nk[d] + off) * (double)(r + g.rmin[d]);
     double s, c;
     sincospi(2 * ph, &s, &c);
-    tw[2 * t] = c;
-    tw[2 * t + 1] = s;
+    reinterpret_cast<double2 *>(tw)[t] = make_double2(c, s);
 }
 
 __global__ void twiddle_kernel(PhaseGeom g, StageInput in, int has_a, int use_shift, double *tw)
@@ -191,9 +190,7 @@ __device__ __forceinline__ void contract_tiled_body(const ContractArgs &a, long 
             if (r < W) acc = cfma(tws[ii * WMAX + r], x[r], acc);
         long dst = (o * a.Nkd + (i0 + ii)) * a.inner + in;
         if (a.out_mode == 1) dst = a.kmap[dst];
-        double *p = a.out + 2 * (dst * a.M + m);
-        p[0] = acc.x;
-        p[1] = acc.y;
+        reinterpret_cast<double2 *>(a.out)[dst * a.M + m] = make_double2(acc.x, acc.y);
     }
 }
 

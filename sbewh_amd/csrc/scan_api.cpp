@@ -502,7 +502,21 @@ int sbe_scan_run(sbe_scan *scan, int occupied_bands, int output_count, double *t
         HIPCHK(hipStreamSynchronize(S.stream));
         std::vector<double> tv((size_t)B * output_count, 0.0);
         int active = B;
+        // SBEWH_PROGRESS=<seconds>: a progress line on stderr at that interval
+        const char *penv = std::getenv("SBEWH_PROGRESS");
+        const double pint = penv ? std::atof(penv) : 0.0;
+        auto plast = std::chrono::steady_clock::now();
+        long iters = 0;
         while (active > 0) {
+            if (pint > 0 && std::chrono::duration<double>(std::chrono::steady_clock::now() - plast).count() > pint) {
+                plast = std::chrono::steady_clock::now();
+                long done_steps = 0;
+                for (const Ctl &q : S.ctl) done_steps += q.steps;
+                fprintf(stderr, "[scan] %.1f s: %d of %d instances in flight, %ld iterations, %ld steps\n",
+                        std::chrono::duration<double>(plast - tic).count(), active, B, iters, done_steps);
+                fflush(stderr);
+            }
+            iters++;
             S.reset_arena();
             // interval boundaries: observe, open the next interval or finish
             std::vector<int> obs, firsts, att;

@@ -65,16 +65,26 @@ template <int N, int LR>
 __device__ __forceinline__ void load_mats(cd *lds, const double *src, long stride, long off, int n, long nk, long blk)
 {
     using G = Geo<N, LR>;
-    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
-        int kk = e / G::NN, idx = e % G::NN;
-        int a = idx / N, b = idx % N;
-        long k = blk * G::KPW + kk;
-        cd v = mk(0, 0);
-        if (a < n && b < n && k < nk) {
-            const double *p = src + 2 * (k * stride + off + a * n + b);
-            v = mk(p[0], p[1]);
+    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
+    constexpr int CH = N >= 16 ? 4 : IT;     // entries whose loads are in flight together
+    const double2 *s2 = reinterpret_cast<const double2 *>(src);
+    // unconditional loads (masked entries read entry `off` of k-point 0)
+#pragma unroll
+    for (int i0 = 0; i0 < IT; i0 += CH) {
+        double2 v[CH];
+        bool ok[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int kk = e / G::NN, idx = e % G::NN;
+            const int a = idx / N, b = idx % N;
+            const long k = blk * G::KPW + kk;
+            ok[u] = a < n && b < n && k < nk;
+            v[u] = s2[ok[u] ? k * stride + off + a * n + b : off];
         }
-        lds[e] = v;
+#pragma unroll
+        for (int u = 0; u < CH; u++)
+            lds[threadIdx.x + 64 * (i0 + u)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
     }
 }
 
@@ -86,14 +96,61 @@ template <int N, int LR>
 __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, long blk)
 {
     using G = Geo<N, LR>;
-    const int n = a.n;
-    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
-        int kk = e / G::NN, idx = e % G::NN;
-        int r = idx / N, c = idx % N;
-        long k = blk * G::KPW + kk;
-        cd v = mk(0, 0);
-        if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, 2 * n * n, op * n * n + r * n + c);
-        lds[e] = v;
+    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
+    constexpr int WQ = 16;                   // largest R extent of the last axis
+    const int n = a.n, nn = n * n, M = 2 * nn, WL = a.ph.WL;
+    if (WL > WQ) {
+        // wide R box along the last axis: one entry at a time
+        for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
+            const int kk = e / G::NN, idx = e % G::NN;
+            const int r = idx / N, c = idx % N;
+            const long k = blk * G::KPW + kk;
+            cd v = mk(0, 0);
+            if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, M, op * nn + r * n + c);
+            lds[e] = v;
+        }
+        return;
+    }
+    const int2 *kph = reinterpret_cast<const int2 *>(a.ph.kphase);
+    const double2 *Yb = reinterpret_cast<const double2 *>(a.ph.Y);
+    const double2 *Tb = reinterpret_cast<const double2 *>(a.ph.twL);
+    // the lane's entries in chunks of CH, all r of a chunk in one unrolled
+    // block, so that the loads of different entries are in flight together
+    constexpr int CH = N >= 16 ? 4 : IT;
+#pragma unroll
+    for (int i0 = 0; i0 < IT; i0 += CH) {
+        cd acc[CH];
+        const double2 *yp[CH];
+        const double2 *tp[CH];
+        bool ok[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int kk = e / G::NN, idx = e % G::NN;
+            const int r = idx / N, c = idx % N;
+            const long k = blk * G::KPW + kk;
+            // masked entries read valid addresses and are zeroed at the end:
+            // the loads stay unconditional.  The k of an entry is uniform over
+            // its lanes when NN >= 64, so is its phase map.
+            const bool krc = r < n && c < n;
+            ok[u] = krc && k < a.nk;
+            const int2 ph = kph[k < a.nk ? k : 0];
+            yp[u] = Yb + (long)ph.x * M + (krc ? op * nn + r * n + c : 0);
+            tp[u] = Tb + ph.y;
+            acc[u] = mk(0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < WQ; q++) {
+            if (q < WL) {
+#pragma unroll
+                for (int u = 0; u < CH; u++) {
+                    const double2 tq = tp[u][q], yq = yp[u][(long)q * M];
+                    acc[u] = cfma(mk(tq.x, tq.y), mk(yq.x, yq.y), acc[u]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lds[threadIdx.x + 64 * (i0 + u)] = ok[u] ? acc[u] : mk(0, 0);
     }
 }
 
@@ -105,25 +162,43 @@ __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
     using G = Geo<N, LR>;
     const int n = a.n;
     const StageInput &in = a.in;
-    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
-        int kk = e / G::NN, idx = e % G::NN;
-        int r = idx / N, c = idx % N;
-        long k = blk * G::KPW + kk;
-        cd v = mk(0, 0);
-        if (r < n && c < n && k < a.nk) {
-            const long i = 8 + k * a.P + r * n + c;
-            const double2 xv = reinterpret_cast<const double2 *>(in.x)[i];
-            double re = xv.x, im = xv.y;
+    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
+    constexpr int CH = N >= 16 ? 4 : IT;     // entries whose loads are in flight together
 #pragma unroll
-            for (int j = 0; j < 6; j++)
-                if (j < in.nt) {
-                    const double2 kv = reinterpret_cast<const double2 *>(in.k[j])[i];
-                    re += in.c[j] * kv.x;
-                    im += in.c[j] * kv.y;
-                }
-            v = mk(re, im);
+    for (int i0 = 0; i0 < IT; i0 += CH) {
+        long idx_[CH];
+        bool ok[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int kk = e / G::NN, idx = e % G::NN;
+            const int r = idx / N, c = idx % N;
+            const long k = blk * G::KPW + kk;
+            ok[u] = r < n && c < n && k < a.nk;
+            // masked entries read state entry 8 (valid) and are zeroed at the end
+            idx_[u] = ok[u] ? 8 + k * a.P + r * n + c : 8;
         }
-        lds[e] = v;
+        double re[CH], im[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const double2 xv = reinterpret_cast<const double2 *>(in.x)[idx_[u]];
+            re[u] = xv.x;
+            im[u] = xv.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+            if (j < in.nt) {
+                const double2 *kj = reinterpret_cast<const double2 *>(in.k[j]);
+                const double cj = in.c[j];
+#pragma unroll
+                for (int u = 0; u < CH; u++) {
+                    const double2 kv = kj[idx_[u]];
+                    re[u] += cj * kv.x;
+                    im[u] += cj * kv.y;
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lds[threadIdx.x + 64 * (i0 + u)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
     }
 }
 
@@ -348,6 +423,23 @@ __device__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg
     }
 }
 
+// eigenvector rows from LDS natural layout into the per-k cache [k][n][n],
+// with the lane -> entry mapping of load_mats
+template <int N, int LR>
+__device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, long blk)
+{
+    using G = Geo<N, LR>;
+    const int n = a.n, nn = n * n;
+#pragma unroll
+    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
+        const int kk = e / G::NN, idx = e % G::NN;
+        const int r = idx / N, c = idx % N;
+        const long k = blk * G::KPW + kk;
+        if (r < n && c < n && k < a.nk)
+            reinterpret_cast<double2 *>(a.ucache)[k * nn + r * n + c] = make_double2(lds[e].x, lds[e].y);
+    }
+}
+
 // the stage for the KPW k-points of logical block blk of one instance
 template <int N, int LR>
 __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
@@ -383,9 +475,6 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     }
 
     cd dr[C];
-#pragma unroll
-    for (int c = 0; c < C; c++) dr[c] = mk(0, 0);
-
     if (a.dephase) {
         cd A[C], U[C];
         if (a.warm) {
@@ -413,24 +502,32 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
         }
         store_xrow<N, LR>(s1, L, U);  // U -> S1
-        load_rho<N, LR>(S0, a, blk);  // rho -> S0
         wsync();
-        cd Rho[C];
-        load_xrow<N, LR>(s0, L, Rho);  // kept for the coherent part
-        if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
-        if (a.ucache) {
-            // keep the eigenvectors for the next call's warm start
-            const long k0 = blk * G::KPW;
-            for (int e = threadIdx.x; e < G::KPW * nn; e += 64) {
-                int kk = e / nn, p = e % nn;
-                long k = k0 + kk;
-                if (k >= a.nk) break;
-                cd v = S1[kk * G::NN + (p / n) * N + (p % n)];
-                double *o = a.ucache + 2 * (k * nn + p);
-                o[0] = v.x;
-                o[1] = v.y;
-            }
-        }
+        // the eigenvectors go to the per-k cache: warm start of the next call,
+        // and the copy the dephasing below reloads (each lane reads back only
+        // the entries it stored itself, same-address order within a thread)
+        store_ucache<N, LR>(a, S1, blk);
+    }
+    load_rho<N, LR>(S0, a, blk);  // rho -> S0
+    wsync();
+    if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
+    // coherent part: -i (H_E rho - rho H_E)
+    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
+    wsync();
+    {
+        cd t1[C];
+        lds_product<N, LR, 0, 0>(s1, s0, L, t1);
+#pragma unroll
+        for (int c = 0; c < C; c++) dr[c] = mk(t1[c].y, -t1[c].x);
+        lds_product<N, LR, 0, 0>(s0, s1, L, t1);
+#pragma unroll
+        for (int c = 0; c < C; c++) dr[c] = mk(dr[c].x - t1[c].y, dr[c].y + t1[c].x);
+    }
+    if (a.dephase) {
+        // dephasing in the eigenbasis: U^dagger (D o (U rho U^dagger)) U
+        wsync();
+        load_mats<N, LR>(S1, a.ucache, nn, 0, n, a.nk, blk);  // U -> S1
+        wsync();
         cd T[C];
         lds_product<N, LR, 0, 0>(s1, s0, L, T);  // T = U rho
         wsync();
@@ -457,27 +554,9 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
         wsync();
         store_xrow<N, LR>(s0, L, T);
         wsync();
-        lds_product<N, LR, 1, 0>(s1, s0, L, dr);  // U^dagger S
-        wsync();
-        store_xrow<N, LR>(s0, L, Rho);
-    } else {
-        load_rho<N, LR>(S0, a, blk);
-        if (a.xout) {
-            wsync();
-            store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
-        }
-    }
-    // coherent part: -i (H_E rho - rho H_E)
-    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
-    wsync();
-    {
-        cd t1[C];
-        lds_product<N, LR, 0, 0>(s1, s0, L, t1);
+        lds_product<N, LR, 1, 0>(s1, s0, L, T);  // U^dagger S
 #pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = mk(dr[c].x + t1[c].y, dr[c].y - t1[c].x);
-        lds_product<N, LR, 0, 0>(s0, s1, L, t1);
-#pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = mk(dr[c].x - t1[c].y, dr[c].y + t1[c].x);
+        for (int c = 0; c < C; c++) dr[c] = cadd(dr[c], T[c]);
     }
 
     // write d rho/dt (with zero pads) through LDS for coalescing
@@ -500,8 +579,8 @@ __global__ __launch_bounds__(64) void rhs_stage_batched_kernel(const StageArgs *
 {
     const long gb = xcd_block(blockIdx.x, gridDim.x);
     const int j = find_entry(block0, m, gb);
-    const StageArgs a = args[j];
-    rhs_stage_body<N, LR>(a, gb - block0[j]);
+    // read in place (a local copy of the 256-byte struct would live in scratch)
+    rhs_stage_body<N, LR>(args[j], gb - block0[j]);
 }
 
 // initial Fermi-Dirac state rho = U^dagger f(eps) U, and/or eigenvalues and

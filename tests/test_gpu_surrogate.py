@@ -44,6 +44,20 @@ def test_forward_matches_oracle():
         assert rel2(y[e], ref) <= 1e-2, (e, rel2(y[e], ref))
 
 
+def test_forward_rows_independent_of_batch_size():
+    """a row's output does not depend on the other rows of the call, also when
+    a call has fewer rows than an earlier one (the buffers keep their capacity)"""
+    say("[gpu] mlp forward batch-size independence")
+    ens = MLPEnsemble(DIMS, 5, seed=44)
+    rng = np.random.default_rng(2)
+    x, _ = batch(rng, 300)
+    full = ens.forward(x)
+    part = ens.forward(x[:5])
+    one = ens.forward(x[7:8])
+    assert np.array_equal(part, full[:, :5])
+    assert np.array_equal(one, full[:, 7:8])
+
+
 @pytest.mark.parametrize("rows", [64, 257])
 def test_gradient_matches_oracle(rows):
     say(f"[gpu] mlp gradient, {rows} rows")
