@@ -148,6 +148,7 @@ struct sbe_ctx {
     double *tw = nullptr, *y1 = nullptr, *y2 = nullptr;
     double *mesh = nullptr;    // full phase sum {H0, H_E} (initial state, Fermi level, hk_mesh)
     double *Y = nullptr;       // partial phase sum of the RHS operators (all axes but the last)
+    double *hm = nullptr;      // model box with the operator merge of the current RHS (dim >= 2)
     double *Yobs = nullptr;    // partial phase sum of the observer operators
     int *kphase = nullptr;     // per k: (rest(k) W_L, i_L(k) W_L)
     int WL = 1;
@@ -247,6 +248,20 @@ struct sbe_ctx {
 
     PhaseBuffers pbufs() const { return PhaseBuffers{Hb, Db, dHb, kmap, tw, y1, y2}; }
     PhaseLast last(const double *Yp) const { return PhaseLast{Yp, tw + 2 * twL_off, kphase, WL}; }
+    // the merged model box of the RHS (for dim == 1 it is the partial phase sum itself)
+    double *merged() const { return dim == 1 ? Y : hm; }
+    ModelMerge merge_of(const double E[3]) const
+    {
+        ModelMerge mm{};
+        mm.Hb = Hb;
+        mm.Db = Db;
+        for (int d = 0; d < 3; d++) mm.E[d] = d < dim ? E[d] : 0.0;
+        mm.nbox = (long)geom.W[0] * geom.W[1] * geom.W[2];
+        mm.nn = nn;
+        mm.dim = dim;
+        mm.out = merged();
+        return mm;
+    }
 
     // dxdt = f(rho_in, t) on device with rho_in = in.x + sum_j in.c[j] in.k[j];
     // xout (optional) receives rho_in
@@ -255,8 +270,8 @@ struct sbe_ctx {
         double E[3];
         E_at(t, E);
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
-        HIPCHK(launch_twiddles(geom, &in, 1, tw, s));
-        HIPCHK(launch_phase_partial(geom, pbufs(), 2, E[0], E[1], E[2], Y, s));
+        HIPCHK(launch_phase_prep(geom, &in, 1, tw, merge_of(E), s));
+        HIPCHK(launch_phase_partial(geom, pbufs(), 2, E[0], E[1], E[2], Y, merged(), s));
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
         StageArgs a{};
         a.in = in;
@@ -293,7 +308,7 @@ struct sbe_ctx {
         StageInput in{};
         in.x = xs;
         HIPCHK(launch_twiddles(geom, &in, 1, tw, s));
-        HIPCHK(launch_phase_partial(geom, pbufs(), obs_ops, 0, 0, 0, Yobs, s));
+        HIPCHK(launch_phase_partial(geom, pbufs(), obs_ops, 0, 0, 0, Yobs, nullptr, s));
         ObsArgs o{};
         o.x = xs;
         o.ph = last(Yobs);
@@ -347,7 +362,7 @@ struct sbe_ctx {
 
     ~sbe_ctx()
     {
-        double *ptrs[] = {Hb, Db, dHb, tw, y1, y2, mesh, Y, Yobs, partial, jbuf, ucache};
+        double *ptrs[] = {Hb, Db, dHb, tw, y1, y2, mesh, Y, hm, Yobs, partial, jbuf, ucache};
         for (double *p : ptrs)
             if (p) (void)hipFree(p);
         for (double *p : st)
@@ -560,6 +575,7 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
     c->y2 = dalloc<double>(2 * ysz);
     c->mesh = dalloc<double>(2 * c->nk * 2 * nn);
     c->Y = dalloc<double>(2 * phase_partial_size(G, 2));
+    if (g->dim > 1) c->hm = dalloc<double>(2 * nbox * 2 * nn);
     c->Yobs = dalloc<double>(2 * phase_partial_size(G, c->obs_ops));
     {
         // per k: Y row offset of rest(k) and twiddle offset of i_L(k), L = dim - 1

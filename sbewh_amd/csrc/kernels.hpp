@@ -35,6 +35,21 @@ struct StageInput {
 // input *in (A = 0 if in == null); use_shift = 0 drops kshift_au.
 hipError_t launch_twiddles(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw, hipStream_t s);
 
+// Operator merge of the model box for the RHS, done once per evaluation:
+//   out[b][e]      = H(R_b)[e]
+//   out[b][nn + e] = (H + sum_{d<dim} E_d r_d)(R_b)[e]      (b < nbox, e < nn)
+// (code/src/WhFourierTransform.cpp:63-66).  out = null: no merge.
+struct ModelMerge {
+    const double *Hb, *Db;
+    double E[3];
+    long nbox;
+    int nn, dim;
+    double *out;
+};
+// twiddles (as launch_twiddles) and the model merge in one launch
+hipError_t launch_phase_prep(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw,
+                             const ModelMerge &mm, hipStream_t s);
+
 // One contraction stage of the separable phase sum (see phase_kernels.hip).
 struct ContractArgs {
     const double *in;    // mode 0: [outer][W][inner][M]
@@ -77,11 +92,13 @@ hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, 
 // ops == 2: {H0, H_E}; ops == 1 + 2 dim: {H0, D_0.., dH0/dk_0..}.
 // The consuming kernel contracts r_L itself.  For dim == 1, Y is the
 // operator-merged model box [W_0][ops nn].
+// merged: the model box merged by launch_phase_prep (ops == 2; for dim == 1 it is
+// Y itself and no stage runs), or null to read the model operators directly.
 hipError_t launch_phase_partial(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
-                                double E2, double *Y, hipStream_t s);
+                                double E2, double *Y, const double *merged, hipStream_t s);
 // the stages of launch_phase_partial (at most 2), returns their count
 int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1, double E2,
-                       double *Y, ContractStage *st);
+                       double *Y, const double *merged, ContractStage *st);
 // complex entries of Y and of the stage scratch it needs (y1)
 long phase_partial_size(const PhaseGeom &g, int ops);
 long phase_partial_scratch(const PhaseGeom &g, int ops);
@@ -105,8 +122,9 @@ struct TwiddleEntry {
     int has_a;
     int use_shift;
     double *tw;
+    ModelMerge mm;
 };
-// grid: ceil(max_count / 256) x m, max_count = max_j sum_d Nk_d W_d
+// grid: ceil(max_count / 256) x m, max_count = max_j (sum_d Nk_d W_d + merged entries)
 hipError_t launch_twiddles_batched(const TwiddleEntry *d_entries, int m, int max_count, hipStream_t s);
 // entry j owns blocks [block0[j], block0[j] + blocks_x * blocks_y); blocks = total
 hipError_t launch_contract_batched(const ContractStage *d_stages, const long *d_block0, int m, long blocks,

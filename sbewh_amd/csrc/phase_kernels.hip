@@ -50,16 +50,47 @@ __device__ __forceinline__ void twiddle_body(const PhaseGeom &g, const StageInpu
     reinterpret_cast<double2 *>(tw)[t] = make_double2(c, s);
 }
 
-__global__ void twiddle_kernel(PhaseGeom g, StageInput in, int has_a, int use_shift, double *tw)
+__device__ __forceinline__ int twiddle_count(const PhaseGeom &g)
 {
-    twiddle_body(g, in, has_a, use_shift, tw, blockIdx.x * blockDim.x + threadIdx.x);
+    return g.Nk[0] * g.W[0] + g.Nk[1] * g.W[1] + g.Nk[2] * g.W[2];
+}
+
+// entry t of the merged model box (see ModelMerge)
+__device__ __forceinline__ void merge_body(const ModelMerge &mm, long t)
+{
+    const long M = 2L * mm.nn;
+    if (!mm.out || t >= mm.nbox * M) return;
+    const long b = t / M;
+    const int m = (int)(t - b * M), op = m >= mm.nn, e = m - op * mm.nn;
+    const double2 h = reinterpret_cast<const double2 *>(mm.Hb)[b * mm.nn + e];
+    double re = h.x, im = h.y;
+    if (op)
+        for (int d = 0; d < mm.dim; d++) {
+            const double2 r = reinterpret_cast<const double2 *>(mm.Db)[((long)d * mm.nbox + b) * mm.nn + e];
+            re += mm.E[d] * r.x;
+            im += mm.E[d] * r.y;
+        }
+    reinterpret_cast<double2 *>(mm.out)[t] = make_double2(re, im);
+}
+
+__device__ __forceinline__ void prep_body(const PhaseGeom &g, const StageInput &in, int has_a, int use_shift,
+                                          double *tw, const ModelMerge &mm, long t)
+{
+    const int ntw = twiddle_count(g);
+    if (t < ntw) twiddle_body(g, in, has_a, use_shift, tw, (int)t);
+    else merge_body(mm, t - ntw);
+}
+
+__global__ void twiddle_kernel(PhaseGeom g, StageInput in, int has_a, int use_shift, double *tw, ModelMerge mm)
+{
+    prep_body(g, in, has_a, use_shift, tw, mm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // one instance per blockIdx.y
 __global__ void twiddle_batched_kernel(const TwiddleEntry *e)
 {
     const TwiddleEntry &q = e[blockIdx.y];
-    twiddle_body(q.g, q.in, q.has_a, q.use_shift, q.tw, blockIdx.x * blockDim.x + threadIdx.x);
+    prep_body(q.g, q.in, q.has_a, q.use_shift, q.tw, q.mm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 constexpr int IB = 8;
@@ -194,21 +225,20 @@ __device__ __forceinline__ void contract_tiled_body(const ContractArgs &a, long 
     }
 }
 
-// operator-merged model box Y[b][m] = op_m(R_b) (dim == 1)
-__device__ __forceinline__ void merge_body(const ContractArgs &a, long bx)
+// operator-merged model box Y[b][m] = op_m(R_b) (dim == 1, observer operators)
+__device__ __forceinline__ void box_body(const ContractArgs &a, long bx)
 {
     long t = bx * 256 + threadIdx.x;
     if (t >= a.nbox * a.M) return;
     cd v = model_value(a, t / a.M, (int)(t % a.M));
-    a.out[2 * t] = v.x;
-    a.out[2 * t + 1] = v.y;
+    reinterpret_cast<double2 *>(a.out)[t] = make_double2(v.x, v.y);
 }
 
 __device__ __forceinline__ void stage_body(const ContractStage &st, long lb)
 {
     if (st.kind == 0) contract_tiled_body(st.a, lb % st.blocks_x, (int)(lb / st.blocks_x));
     else if (st.kind == 1) contract_body(st.a, lb);
-    else merge_body(st.a, lb);
+    else box_body(st.a, lb);
 }
 
 __global__ __launch_bounds__(256) void contract_stage_kernel(ContractStage st)
@@ -225,13 +255,21 @@ __global__ __launch_bounds__(256) void contract_batched_kernel(const ContractSta
     stage_body(q, gb - block0[j]);
 }
 
-hipError_t launch_twiddles(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw, hipStream_t s)
+hipError_t launch_phase_prep(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw,
+                             const ModelMerge &mm, hipStream_t s)
 {
-    int tot = g.Nk[0] * g.W[0] + g.Nk[1] * g.W[1] + g.Nk[2] * g.W[2];
+    long tot = (long)g.Nk[0] * g.W[0] + (long)g.Nk[1] * g.W[1] + (long)g.Nk[2] * g.W[2];
+    if (mm.out) tot += mm.nbox * 2L * mm.nn;
     StageInput v{};
     if (in) v = *in;
-    hipLaunchKernelGGL(twiddle_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, g, v, in ? 1 : 0, use_shift, tw);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, v, in ? 1 : 0,
+                       use_shift, tw, mm);
     return hipGetLastError();
+}
+
+hipError_t launch_twiddles(const PhaseGeom &g, const StageInput *in, int use_shift, double *tw, hipStream_t s)
+{
+    return launch_phase_prep(g, in, use_shift, tw, ModelMerge{}, s);
 }
 
 long twiddle_offset(const PhaseGeom &g, int d)
@@ -341,7 +379,7 @@ long phase_partial_scratch(const PhaseGeom &g, int ops)
 }
 
 int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1, double E2,
-                       double *Y, ContractStage *st)
+                       double *Y, const double *merged, ContractStage *st)
 {
     const int nn = g.n * g.n;
     ContractArgs a{};
@@ -358,6 +396,7 @@ int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, doubl
     const int model_mode = ops == 2 ? 1 : 2;
     a.in_mode = model_mode;
     const int L = g.dim - 1;
+    if (L == 0 && merged) return 0;  // the merged box is Y
     if (L == 0) {
         a.out = Y;
         st[0] = ContractStage{};
@@ -372,8 +411,8 @@ int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, doubl
         long outer = 1, inner = 1;
         for (int e = 0; e < d; e++) outer *= g.Nk[e];
         for (int e = d + 1; e < 3; e++) inner *= g.W[e];
-        a.in = src;
-        a.in_mode = d == 0 ? model_mode : 0;
+        a.in = (d == 0 && merged) ? merged : src;
+        a.in_mode = (d == 0 && !merged) ? model_mode : 0;
         a.W = g.W[d];
         a.Nkd = g.Nk[d];
         a.outer = outer;
@@ -388,10 +427,10 @@ int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, doubl
 }
 
 hipError_t launch_phase_partial(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
-                                double E2, double *Y, hipStream_t s)
+                                double E2, double *Y, const double *merged, hipStream_t s)
 {
     ContractStage st[2];
-    int ns = phase_partial_plan(g, b, ops, E0, E1, E2, Y, st);
+    int ns = phase_partial_plan(g, b, ops, E0, E1, E2, Y, merged, st);
     for (int i = 0; i < ns; i++) {
         launch_stage(st[i], s);
         hipError_t e = hipGetLastError();

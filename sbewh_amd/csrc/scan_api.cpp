@@ -47,7 +47,7 @@ struct sbe_scan {
     std::string err;
     std::vector<sbe_ctx *> inst;
     std::vector<Ctl> ctl;
-    int n = 0, N = 0, dim = 0, ns = 0;
+    int n = 0, N = 0, dim = 0, ns = 0, ns_obs = 0;  // contraction stages of the RHS / observer sums
     hipStream_t stream = nullptr;
     // stage layout shared by all instances (tuned on the first batched RHS)
     int lr = 0;
@@ -105,12 +105,14 @@ struct sbe_scan {
     void reset_arena() { arena_used = arena_flushed = 0; }
 
     // ---- batched kernels ----------------------------------------------------
-    // twiddles of every request (A of its stage input)
-    void twiddles(const std::vector<std::pair<sbe_ctx *, StageInput>> &rq)
+    // twiddles of every request (A of its stage input) and, with fields E,
+    // the operator merge of the model box
+    void twiddles(const std::vector<std::pair<sbe_ctx *, StageInput>> &rq,
+                  const std::vector<std::array<double, 3>> *E)
     {
         const int m = (int)rq.size();
         std::vector<TwiddleEntry> tw(m);
-        int mx = 0;
+        long mx = 0;
         for (int j = 0; j < m; j++) {
             const sbe_ctx &c = *rq[j].first;
             tw[j].g = c.geom;
@@ -118,25 +120,30 @@ struct sbe_scan {
             tw[j].has_a = 1;
             tw[j].use_shift = 1;
             tw[j].tw = c.tw;
-            mx = std::max(mx, c.geom.Nk[0] * c.geom.W[0] + c.geom.Nk[1] * c.geom.W[1] + c.geom.Nk[2] * c.geom.W[2]);
+            tw[j].mm = E ? c.merge_of((*E)[j].data()) : ModelMerge{};
+            long cnt = (long)c.geom.Nk[0] * c.geom.W[0] + (long)c.geom.Nk[1] * c.geom.W[1] +
+                       (long)c.geom.Nk[2] * c.geom.W[2];
+            if (tw[j].mm.out) cnt += tw[j].mm.nbox * 2L * tw[j].mm.nn;
+            mx = std::max(mx, cnt);
         }
         TwiddleEntry *d = add(tw.data(), m);
         flush();
-        HIPCHK(launch_twiddles_batched(d, m, mx, stream));
+        HIPCHK(launch_twiddles_batched(d, m, (int)mx, stream));
     }
     // partial phase sums (ops 2 with per-request field E, or observer ops)
     void partial(const std::vector<sbe_ctx *> &cs, const std::vector<std::array<double, 3>> &E, int ops, bool obs)
     {
         const int m = (int)cs.size();
-        std::vector<std::vector<ContractStage>> st(ns, std::vector<ContractStage>(m));
+        std::vector<std::vector<ContractStage>> st(2, std::vector<ContractStage>(m));
         for (int j = 0; j < m; j++) {
             ContractStage tmp[2];
             sbe_ctx &c = *cs[j];
-            int k = phase_partial_plan(c.geom, c.pbufs(), ops, E[j][0], E[j][1], E[j][2], obs ? c.Yobs : c.Y, tmp);
-            if (k != ns) fail(SBE_ERR_ARG, "inconsistent phase-sum plans in the scan");
-            for (int d = 0; d < ns; d++) st[d][j] = tmp[d];
+            int k = phase_partial_plan(c.geom, c.pbufs(), ops, E[j][0], E[j][1], E[j][2], obs ? c.Yobs : c.Y,
+                                       obs ? nullptr : c.merged(), tmp);
+            if (k != (obs ? ns_obs : ns)) fail(SBE_ERR_ARG, "inconsistent phase-sum plans in the scan");
+            for (int d = 0; d < k; d++) st[d][j] = tmp[d];
         }
-        for (int d = 0; d < ns; d++) {
+        for (int d = 0; d < (obs ? ns_obs : ns); d++) {
             std::vector<long> b0(m);
             long tot = 0;
             for (int j = 0; j < m; j++) {
@@ -163,7 +170,7 @@ struct sbe_scan {
             cs[j] = rq[j].c;
             rq[j].c->E_at(rq[j].t, E[j].data());
         }
-        twiddles(tw);
+        twiddles(tw, &E);
         partial(cs, E, 2, false);
         std::vector<StageArgs> args(m);
         for (int j = 0; j < m; j++) {
@@ -281,7 +288,7 @@ struct sbe_scan {
             b0[j] = tot;
             tot += c.nk;
         }
-        twiddles(tw);
+        twiddles(tw, nullptr);
         partial(cs, E, inst[ids[0]]->obs_ops, true);
         ObsArgs *da = add(oa.data(), m);
         long *db = add(b0.data(), m);
@@ -433,7 +440,8 @@ int sbe_scan_create(const sbe_model *model, const sbe_grid *grid, const sbe_para
         sc->n = sc->inst[0]->n;
         sc->N = sc->inst[0]->N;
         sc->dim = sc->inst[0]->dim;
-        sc->ns = sc->dim > 1 ? sc->dim - 1 : 1;
+        sc->ns = sc->dim - 1;                        // the merged box feeds the first contraction
+        sc->ns_obs = sc->dim > 1 ? sc->dim - 1 : 1;  // dim 1: one operator-box pass
         sc->arena_cap = 12 * arena_bytes_per_instance() * (size_t)count + (1 << 20);
         HIPCHK(hipHostMalloc((void **)&sc->h_arena, sc->arena_cap, hipHostMallocDefault));
         HIPCHK(hipMalloc((void **)&sc->d_arena, sc->arena_cap));

@@ -56,6 +56,23 @@ __device__ __forceinline__ Lane lane_id(long nk, long blk)
     return L;
 }
 
+// LDS position of element (a, b) of an N x N matrix: row a, column b ^ g(a),
+// with g(a) = a * x in GF(2^m) (N = 2^m; x^2+x+1, x^3+x+1, x^4+x+1).  g and
+// a -> a ^ g(a) are bijections, so the wavefront's column walks (X[i][l] over
+// rows i), row walks (Y[l][j]) and XOR-wrapped row accesses (row i, column
+// i ^ c) of one k-point all fall on distinct LDS banks.
+template <int N>
+__device__ __forceinline__ int sw(int a, int b)
+{
+    if constexpr (N == 2) {
+        return a * N + b;
+    } else {
+        // reduction by x^m = x + 1: the same low bits (3) for m = 2, 3, 4
+        const int g = ((a << 1) & (N - 1)) ^ ((a & (N >> 1)) ? 3 : 0);
+        return a * N + (b ^ g);
+    }
+}
+
 // wave-local sync: blocks are exactly one wavefront
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
@@ -83,8 +100,11 @@ __device__ __forceinline__ void load_mats(cd *lds, const double *src, long strid
             v[u] = s2[ok[u] ? k * stride + off + a * n + b : off];
         }
 #pragma unroll
-        for (int u = 0; u < CH; u++)
-            lds[threadIdx.x + 64 * (i0 + u)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int idx = e % G::NN;
+            lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
+        }
     }
 }
 
@@ -107,7 +127,7 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
             const long k = blk * G::KPW + kk;
             cd v = mk(0, 0);
             if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, M, op * nn + r * n + c);
-            lds[e] = v;
+            lds[kk * G::NN + sw<N>(r, c)] = v;
         }
         return;
     }
@@ -150,7 +170,11 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
             }
         }
 #pragma unroll
-        for (int u = 0; u < CH; u++) lds[threadIdx.x + 64 * (i0 + u)] = ok[u] ? acc[u] : mk(0, 0);
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int idx = e % G::NN;
+            lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? acc[u] : mk(0, 0);
+        }
     }
 }
 
@@ -198,7 +222,11 @@ __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
                 }
             }
 #pragma unroll
-        for (int u = 0; u < CH; u++) lds[threadIdx.x + 64 * (i0 + u)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int idx = e % G::NN;
+            lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
+        }
     }
 }
 
@@ -215,7 +243,7 @@ __device__ __forceinline__ void store_state(double *dst, const cd *lds, int n, i
         long k = k0 + kk;
         if (k >= nk) break;
         cd v = mk(0, 0);
-        if (p < nn) v = lds[kk * G::NN + (p / n) * N + (p % n)];
+        if (p < nn) v = lds[kk * G::NN + sw<N>(p / n, p % n)];
         reinterpret_cast<double2 *>(dst)[8 + k * P + p] = make_double2(v.x, v.y);
     }
 }
@@ -230,11 +258,11 @@ __device__ __forceinline__ void lds_product(const cd *X, const cd *Y, const Lane
     for (int c = 0; c < G::C; c++) out[c] = mk(0, 0);
 #pragma unroll 2
     for (int l = 0; l < N; l++) {
-        cd x = (XOP == 0) ? X[L.i * N + l] : cconj(X[l * N + L.i]);
+        cd x = (XOP == 0) ? X[sw<N>(L.i, l)] : cconj(X[sw<N>(l, L.i)]);
 #pragma unroll
         for (int c = 0; c < G::C; c++) {
             int j = L.i ^ (L.h * G::C + c);
-            cd y = (YOP == 0) ? Y[l * N + j] : cconj(Y[j * N + l]);
+            cd y = (YOP == 0) ? Y[sw<N>(l, j)] : cconj(Y[sw<N>(j, l)]);
             out[c] = cfma(x, y, out[c]);
         }
     }
@@ -245,7 +273,7 @@ __device__ __forceinline__ void store_xrow(cd *lds, const Lane &L, const cd *v)
 {
     using G = Geo<N, LR>;
 #pragma unroll
-    for (int c = 0; c < G::C; c++) lds[L.i * N + (L.i ^ (L.h * G::C + c))] = v[c];
+    for (int c = 0; c < G::C; c++) lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))] = v[c];
 }
 
 template <int N, int LR>
@@ -253,7 +281,7 @@ __device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
 {
     using G = Geo<N, LR>;
 #pragma unroll
-    for (int c = 0; c < G::C; c++) v[c] = lds[L.i * N + (L.i ^ (L.h * G::C + c))];
+    for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))];
 }
 
 __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s)
@@ -435,8 +463,10 @@ __device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, 
         const int kk = e / G::NN, idx = e % G::NN;
         const int r = idx / N, c = idx % N;
         const long k = blk * G::KPW + kk;
-        if (r < n && c < n && k < a.nk)
-            reinterpret_cast<double2 *>(a.ucache)[k * nn + r * n + c] = make_double2(lds[e].x, lds[e].y);
+        if (r < n && c < n && k < a.nk) {
+            const cd v = lds[kk * G::NN + sw<N>(r, c)];
+            reinterpret_cast<double2 *>(a.ucache)[k * nn + r * n + c] = make_double2(v.x, v.y);
+        }
     }
 }
 
@@ -642,7 +672,7 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
         long k = k0 + kk;
         if (k >= a.nk) break;
         cd v = mk(0, 0);
-        if (p < nn) v = S0[kk * G::NN + (p / n) * N + (p % n)];
+        if (p < nn) v = S0[kk * G::NN + sw<N>(p / n, p % n)];
         double *o = a.state + 2 * (8 + k * P + p);
         o[0] = v.x;
         o[1] = v.y;
