@@ -73,6 +73,19 @@ __device__ __forceinline__ int sw(int a, int b)
     }
 }
 
+// f(integral_constant<int, c * CH>) for c = 0 .. IT/CH - 1, unrolled at compile
+// time (arrays indexed by the chunk offset stay in registers)
+template <int CH, class F, int... I>
+__device__ __forceinline__ void chunks_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I * CH>{}), ...);
+}
+template <int IT, int CH, class F>
+__device__ __forceinline__ void for_chunks(F &&f)
+{
+    chunks_impl<CH>(f, std::make_integer_sequence<int, IT / CH>{});
+}
+
 // wave-local sync: blocks are exactly one wavefront
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
@@ -86,8 +99,8 @@ __device__ __forceinline__ void load_mats(cd *lds, const double *src, long strid
     constexpr int CH = N >= 16 ? 4 : IT;     // entries whose loads are in flight together
     const double2 *s2 = reinterpret_cast<const double2 *>(src);
     // unconditional loads (masked entries read entry `off` of k-point 0)
-#pragma unroll
-    for (int i0 = 0; i0 < IT; i0 += CH) {
+    for_chunks<IT, CH>([&](auto i0c) {
+        constexpr int i0 = decltype(i0c)::value;
         double2 v[CH];
         bool ok[CH];
 #pragma unroll
@@ -105,7 +118,7 @@ __device__ __forceinline__ void load_mats(cd *lds, const double *src, long strid
             const int idx = e % G::NN;
             lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
         }
-    }
+    });
 }
 
 // H(k) of operator op (0: H0, 1: H_E) for the wave's k-points into LDS
@@ -137,8 +150,8 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
     // the lane's entries in chunks of CH, all r of a chunk in one unrolled
     // block, so that the loads of different entries are in flight together
     constexpr int CH = N >= 16 ? 4 : IT;
-#pragma unroll
-    for (int i0 = 0; i0 < IT; i0 += CH) {
+    for_chunks<IT, CH>([&](auto i0c) {
+        constexpr int i0 = decltype(i0c)::value;
         cd acc[CH];
         const double2 *yp[CH];
         const double2 *tp[CH];
@@ -175,7 +188,7 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
             const int idx = e % G::NN;
             lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? acc[u] : mk(0, 0);
         }
-    }
+    });
 }
 
 // stage input rho = x + sum_j c_j k_j (the Dormand-Prince stage combination,
@@ -188,8 +201,8 @@ __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
     const StageInput &in = a.in;
     constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
     constexpr int CH = N >= 16 ? 4 : IT;     // entries whose loads are in flight together
-#pragma unroll
-    for (int i0 = 0; i0 < IT; i0 += CH) {
+    for_chunks<IT, CH>([&](auto i0c) {
+        constexpr int i0 = decltype(i0c)::value;
         long idx_[CH];
         bool ok[CH];
 #pragma unroll
@@ -227,7 +240,7 @@ __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
             const int idx = e % G::NN;
             lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
         }
-    }
+    });
 }
 
 // the wave's matrices from LDS natural layout into a state vector (reference
@@ -421,7 +434,7 @@ __device__ __forceinline__ void jac_sweep(std::integer_sequence<int, Rm1...>, cd
 // Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (rows), eigenvalues
 // in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Rp/Rq[KPW*N], Fa[KPW*N].
 template <int N, int LR>
-__device__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Rp, cd *Rq, int *Fa,
+__device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Rp, cd *Rq, int *Fa,
                             bool initU = true)
 {
     using G = Geo<N, LR>;
