@@ -156,9 +156,9 @@ struct sbe_ctx {
     double *st[10] = {};  // x, k1..k7, scratch, xn
     double *ucache = nullptr;  // [k][n*n] eigenvectors of the last RHS (warm starts)
     bool ucache_valid = false;
-    unsigned long long *err_bits = nullptr;
     double *err_partial = nullptr;
-    unsigned long long *err_host = nullptr;  // pinned
+    unsigned long long *err_host = nullptr;      // pinned, mapped, fine-grained
+    unsigned long long *err_host_dev = nullptr;  // its device address
     double *partial = nullptr;
     double *jbuf = nullptr;
     long jbuf_cap = 0;
@@ -346,11 +346,34 @@ struct sbe_ctx {
         rhs_dev(input({k1, k3, k4, k5, k6}, {C1, C3, C4, C5, C6}), k7, t + dt, s, xn);
         const double *ek[6] = {k1, k3, k4, k5, k6, k7};
         double e[6] = {dt * DC1, dt * DC3, dt * DC4, dt * DC5, dt * DC6, dt * DC7};
-        HIPCHK(launch_error_norm(S, X, ek, e, prm.eps_abs, prm.eps_rel, dt, err_partial, err_bits, s));
-        HIPCHK(hipMemcpyAsync(err_host, err_bits, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        *reinterpret_cast<volatile unsigned long long *>(err_host) = ERR_PENDING;
+        HIPCHK(launch_error_norm(S, X, ek, e, prm.eps_abs, prm.eps_rel, dt, err_partial, err_host_dev, s));
+        return wait_error(s);
+    }
+
+    // The error norm's final kernel stores the norm into mapped host memory;
+    // poll it instead of waiting for the stream to drain (the next attempt's
+    // launches then queue behind the retiring kernel).  A stream that has
+    // completed, or failed, without the value arriving is an error.
+    static constexpr unsigned long long ERR_PENDING = ~0ull;  // a negative NaN: never a max of |.| terms
+    double wait_error(hipStream_t s)
+    {
+        const volatile unsigned long long *p = err_host;
+        unsigned long long b;
+        for (unsigned it = 1;; it++) {
+            b = *p;
+            if (b != ERR_PENDING) break;
+            if ((it & 4095) == 0) {
+                const hipError_t q = hipStreamQuery(s);
+                if (q == hipErrorNotReady) continue;
+                HIPCHK(q);
+                b = *p;
+                if (b == ERR_PENDING) throw HipFail{hipErrorUnknown, "error norm not delivered"};
+                break;
+            }
+        }
         double v;
-        std::memcpy(&v, err_host, sizeof v);
+        std::memcpy(&v, &b, sizeof v);
         return v;
     }
 
@@ -369,7 +392,6 @@ struct sbe_ctx {
             if (p) (void)hipFree(p);
         if (kmap) (void)hipFree(kmap);
         if (kphase) (void)hipFree(kphase);
-        if (err_bits) (void)hipFree(err_bits);
         if (err_partial) (void)hipFree(err_partial);
         if (err_host) (void)hipHostFree(err_host);
         for (auto &e : ev)
@@ -598,10 +620,12 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
         HIPCHK(hipMemset(p, 0, 16 * c->S));
     }
     c->ucache = dalloc<double>(2 * c->nk * nn);
-    c->err_bits = dalloc<unsigned long long>(1);
     c->err_partial = dalloc<double>(1024);
-    if (!member_of_scan)
-        HIPCHK(hipHostMalloc((void **)&c->err_host, sizeof(unsigned long long), hipHostMallocDefault));
+    if (!member_of_scan) {
+        HIPCHK(hipHostMalloc((void **)&c->err_host, sizeof(unsigned long long),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostGetDevicePointer((void **)&c->err_host_dev, c->err_host, 0));
+    }
     c->partial = dalloc<double>(3 * c->nk);
 }
 

@@ -174,10 +174,11 @@ struct InitArgs {
 hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s);
 
 // ---- RK stage algebra (rk_kernels.hip) -----------------------------------
-// err_max = max_i |sum_j e[j] k[j]_i| / (ea + er (|x_i| + dt |k0_i|)), written as the bit
-// pattern of a non-negative double into *err_bits; partial holds >= 1024 doubles.
+// err_max = max_i |sum_j e[j] k[j]_i| / (ea + er (|x_i| + dt |k0_i|)), stored as the bit
+// pattern of a non-negative double into *out (device memory or the device alias of
+// mapped host memory) with a system-scope store; partial holds >= 1024 doubles.
 hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
-                             double er, double dt, double *partial, unsigned long long *err_bits, hipStream_t s);
+                             double er, double dt, double *partial, unsigned long long *out, hipStream_t s);
 
 // per-instance error norms: entry j covers state S_j of its instance and
 // writes its norm to *err_out (device); partial has >= sum_j blocks_j doubles

@@ -20,7 +20,6 @@ struct ErrArgs {
     double ea, er, dt;
     long S;
     double *partial;      // [ERR_BLOCKS]
-    unsigned long long *err_bits;
 };
 
 constexpr int ERR_BLOCKS = 1024;
@@ -63,6 +62,19 @@ __global__ __launch_bounds__(256) void error_kernel(ErrArgs a)
     if (threadIdx.x == 0) a.partial[blockIdx.x] = v;
 }
 
+// pass 2: one block reduces the partials (the kernel boundary orders them)
+// and stores the bit pattern of the max with a system-scope store, so that a
+// host polling mapped memory sees it without waiting for the stream to drain
+__global__ __launch_bounds__(256) void error_final_kernel(const double *partial, int nb, unsigned long long *out)
+{
+    double v = 0;
+    for (int i = threadIdx.x; i < nb; i += 256) v = fmax(v, partial[i]);
+    v = block_max(v);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(out, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // batched pass 1: block lb of entry j covers entries [lb, lb + 1) * ERR_CHUNK of its state
 __global__ __launch_bounds__(256) void error_batched_kernel(const ErrEntry *e, const long *block0, int m,
                                                             double *partial)
@@ -90,17 +102,8 @@ __global__ __launch_bounds__(256) void error_batched_final_kernel(const ErrEntry
     if (threadIdx.x == 0) *e[j].err_out = v;
 }
 
-// pass 2: one block reduces the partials; writes the bit pattern of the max
-__global__ __launch_bounds__(256) void error_final_kernel(const double *partial, int nb, unsigned long long *err_bits)
-{
-    double v = 0;
-    for (int i = threadIdx.x; i < nb; i += 256) v = fmax(v, partial[i]);
-    v = block_max(v);
-    if (threadIdx.x == 0) *err_bits = (unsigned long long)__double_as_longlong(v);
-}
-
 hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
-                             double er, double dt, double *partial, unsigned long long *err_bits, hipStream_t s)
+                             double er, double dt, double *partial, unsigned long long *out, hipStream_t s)
 {
     ErrArgs a{};
     a.x = x;
@@ -113,14 +116,13 @@ hipError_t launch_error_norm(long S, const double *x, const double *const *k, co
     a.dt = dt;
     a.S = S;
     a.partial = partial;
-    a.err_bits = err_bits;
     long need = (S + 255) / 256;
     int nb = (int)(need < ERR_BLOCKS ? need : ERR_BLOCKS);
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(error_kernel, dim3(nb), dim3(256), 0, s, a);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, partial, nb, err_bits);
+    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
     return hipGetLastError();
 }
 

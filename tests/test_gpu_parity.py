@@ -96,6 +96,26 @@ def test_rhs_no_dephasing_step_mask_cold_jacobi():
             assert rel(g.rhs(x, t), o.rhs(x, t)) < 1e-12
 
 
+def check_error_norms(rg, ro):
+    """the GPU's error norm of every replayed attempt against the oracle's
+    norm of the same accepted attempt (same t, dt; states agree to ~1e-10).
+    The norm is a max of cancelling differences: where it is tiny (no field)
+    its relative precision is poor, so the bound is absolute in units of the
+    acceptance threshold 1, plus relative where the norm decides the step."""
+    lo, lg = ro["log"].as_dict(), rg["log"].as_dict()
+    eo = lo["err"][lo["accepted"] == 1]
+    eg = lg["err"]
+    assert len(eg) == len(eo)
+    d = np.abs(eg - eo)
+    r = d / eo
+    big = eo >= 1e-2
+    i = int(np.argmax(r))
+    say(f"  error norm: abs diff max {d.max():.2e}; rel diff median {np.median(r):.2e}, "
+        f"max {r[i]:.2e} at norm {eo[i]:.2e}; rel max where norm >= 1e-2: {r[big].max():.2e} ({big.sum()} of {len(eo)})")
+    assert d.max() < 1e-6
+    assert r[big].max() < 1e-4
+
+
 def test_replay_chain_full_pulse():
     """full 8-cycle propagation of the 1-D chain: the GPU replays the oracle's
     accepted dt sequence; j(t) and rho agree to 1e-10; the GPU's own adaptive
@@ -114,6 +134,7 @@ def test_replay_chain_full_pulse():
     scale = np.abs(ro["j"]).max()
     assert np.abs(rg["j"] - ro["j"]).max() < 1e-10 * scale
     assert rel(sg, so) < 1e-10
+    check_error_norms(rg, ro)
     # own step control
     sa = st0.copy()
     ra = g.integrate(sa, cfg.output_count)
@@ -139,3 +160,4 @@ def test_replay_square_short_pulse():
     assert rg["rhs_calls"] == 6 * ro["steps"] + cfg.output_count - 1
     assert np.abs(rg["j"] - ro["j"]).max() < 1e-10 * scale
     assert rel(sg, so) < 1e-10
+    check_error_norms(rg, ro)
