@@ -311,21 +311,19 @@ __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s
     }
 }
 
-// column update of one element a (column j) with its partner d (column j ^ r)
-__device__ __forceinline__ cd col_update(cd a, cd d, bool jp, cd cpp, cd cpq)
-{
-    if (jp) {
-        // A[:,p] = (1 + Rqq) A[:,p] - Rqp A[:,q],  Rqq = conj(Rpp), Rqp = -conj(Rpq)
-        const cd cqq = cconj(cpp), cqp = mk(-cpq.x, cpq.y);
-        return csub(cadd(a, cmul(cqq, a)), cmul(cqp, d));
-    }
-    // A[:,q] = -Rpq A[:,p] + (1 + Rpp) A[:,q]
-    return csub(cadd(a, cmul(cpp, a)), cmul(cpq, d));
-}
+// conj(a) * b
+__device__ __forceinline__ cd cmul_cj(cd a, cd b) { return cd{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
 
-// One round r of the parallel Jacobi sweep: rotations on the pairs (p, p ^ r).
+// One round r of the parallel Jacobi sweep: rotations on the pairs (p, q = p ^ r).
+// The rotation J of a pair acts on rows, row p <- J_pp row p + J_pq row q and
+// row q <- J_qq row q + J_qp row p, and as J^dagger on columns,
+// column j <- conj(J_jj) column j + conj(J_j,j^r) column j ^ r.  The owner of
+// element (p, q) stores (J_jj, J_j,j^r) under index j for j = p and j = q
+// (the identity when the pair does not rotate), so every lane reads its
+// row's and its columns' coefficients without selects or divergent branches;
+// an identity coefficient pair reproduces its operand exactly.
 template <int N, int LR, int R>
-__device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double thr, double *Dg, cd *Rp, cd *Rq,
+__device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double thr, double *Dg, cd *Ja, cd *Jb,
                                           int *Fa, int &anyrot)
 {
     using G = Geo<N, LR>;
@@ -333,44 +331,38 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
     constexpr int rc = R % C, rh = R / C;
     constexpr int mrow = R * LR + rh;   // lane mask of the row partner (row i ^ R)
     const int base = L.kk * N;
-    // (a) rotation parameters, computed by the owner of element (p, p^R), p < p^R
+    // (a) rotation of the pair, computed by the owner of element (p, p^R), p < p^R
     if (L.h == rh && L.i < (L.i ^ R)) {
-        cd apq = A[rc];
-        double app = Dg[base + L.i], aqq = Dg[base + (L.i ^ R)];
-        double mod = hypot(apq.x, apq.y);
-        int act = L.valid && (mod > thr);
-        cd rpp = mk(0, 0), rpq = mk(0, 0);
+        const int q = L.i ^ R;
+        const cd apq = A[rc];
+        const double app = Dg[base + L.i], aqq = Dg[base + q];
+        const double mod = hypot(apq.x, apq.y);
+        const int act = L.valid && (mod > thr);
+        cd jpp = mk(1, 0), jpq = mk(0, 0), jqq = mk(1, 0), jqp = mk(0, 0);
         if (act) {
             double c1, s1, c2, s2;
             cs_half(-apq.x, apq.y, c1, s1);
             cs_half(2 * mod, app - aqq, c2, s2);
-            rpp = mk(-s1 * s2 - 1, -c1 * s2);
-            rpq = mk(-c1 * c2, -s1 * c2);
+            jpp = mk(-s1 * s2, -c1 * s2);
+            jpq = mk(-c1 * c2, -s1 * c2);
+            jqq = mk(-s1 * s2, c1 * s2);   // conj(J_pp)
+            jqp = mk(c1 * c2, -s1 * c2);   // -conj(J_pq)
         }
-        Rp[base + L.i] = rpp;
-        Rq[base + L.i] = rpq;
+        Ja[base + L.i] = jpp;
+        Jb[base + L.i] = jpq;
+        Ja[base + q] = jqq;
+        Jb[base + q] = jqp;
         Fa[base + L.i] = act;
     }
     wsync();
-    const int prow = min(L.i, L.i ^ R);
-    const cd rpp = Rp[base + prow], rpq = Rq[base + prow];
-    const int arow = Fa[base + prow];
-    const bool isp = L.i < (L.i ^ R);
+    const int arow = Fa[base + min(L.i, L.i ^ R)];
     anyrot |= arow;
     if (!__any(arow)) {
         // no rotation in this round anywhere in the wavefront: A, U unchanged
         wsync();
         return;
     }
-    // row-pair coefficients: new = ca * x_self + cb * x_partner
-    cd ca, cb;
-    if (isp) {
-        ca = mk(1 + rpp.x, rpp.y);
-        cb = rpq;
-    } else {
-        ca = mk(1 + rpp.x, -rpp.y);  // 1 + Rqq
-        cb = mk(-rpq.x, rpq.y);      // Rqp
-    }
+    const cd ca = Ja[base + L.i], cb = Jb[base + L.i];
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const int c2 = c ^ rc;
@@ -379,15 +371,14 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         cd a0 = A[c], a1 = A[c2];
         const cd b0 = xchg<mrow>(A[c2]), b1 = xchg<mrow>(A[c]);
         const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
-        if (arow) {
-            a0 = cfma(cb, b0, cmul(ca, a0));
-            U[c] = cfma(cb, v0, cmul(ca, U[c]));
-            if (c2 != c) {
-                a1 = cfma(cb, b1, cmul(ca, a1));
-                U[c2] = cfma(cb, v1, cmul(ca, U[c2]));
-            }
+        a0 = cfma(cb, b0, cmul(ca, a0));
+        U[c] = cfma(cb, v0, cmul(ca, U[c]));
+        if (c2 != c) {
+            a1 = cfma(cb, b1, cmul(ca, a1));
+            U[c2] = cfma(cb, v1, cmul(ca, U[c2]));
+        } else {
+            a1 = a0;
         }
-        if (c2 == c) a1 = a0;
         // column update: partner of column j is column j ^ R, register c ^ rc
         cd d0, d1;
         if constexpr (rh == 0) {
@@ -398,25 +389,23 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
             d1 = xchg<rh>(a0);
         }
         const int g0 = L.h * C + c, g1 = L.h * C + c2;
-        const int j0 = L.i ^ g0, j1 = L.i ^ g1;
-        const int pc0 = min(j0, j0 ^ R);
-        const int ac0 = Fa[base + pc0];
-        const cd p0 = Rp[base + pc0], q0 = Rq[base + pc0];
-        cd r0 = ac0 ? col_update(a0, d0, j0 < (j0 ^ R), p0, q0) : a0;
-        if (ac0 && g0 == R) r0 = mk(0, 0);
+        const int j0 = L.i ^ g0;
+        const cd ja = Ja[base + j0], jb = Jb[base + j0];
+        // the rotated pivot (i, i ^ R) is set to zero
+        cd r0 = cfma_cj(jb, d0, cmul_cj(ja, a0));
+        if (g0 == R && arow) r0 = mk(0, 0);
         A[c] = r0;
         if (c2 != c) {
             cd r1;
             if constexpr (rh == 0) {
-                // same column pair as c
-                r1 = ac0 ? col_update(a1, d1, j1 < (j1 ^ R), p0, q0) : a1;
-                if (ac0 && g1 == R) r1 = mk(0, 0);
+                // column j1 = j0 ^ R is j0's partner: (J_j1j1, J_j1j0) = (conj(J_j0j0), -conj(J_j0j1))
+                r1 = cfma(mk(-jb.x, -jb.y), d1, cmul(ja, a1));
             } else {
-                const int pc1 = min(j1, j1 ^ R);
-                const int ac1 = Fa[base + pc1];
-                r1 = ac1 ? col_update(a1, d1, j1 < (j1 ^ R), Rp[base + pc1], Rq[base + pc1]) : a1;
-                if (ac1 && g1 == R) r1 = mk(0, 0);
+                // column j1 = i ^ g1 belongs to another pair (its partner is in lane h ^ rh)
+                const int j1 = L.i ^ g1;
+                r1 = cfma_cj(Jb[base + j1], d1, cmul_cj(Ja[base + j1], a1));
             }
+            if (g1 == R && arow) r1 = mk(0, 0);
             A[c2] = r1;
         }
     }
@@ -426,15 +415,15 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
 
 template <int N, int LR, int... Rm1>
 __device__ __forceinline__ void jac_sweep(std::integer_sequence<int, Rm1...>, cd *A, cd *U, const Lane &L, double thr,
-                                          double *Dg, cd *Rp, cd *Rq, int *Fa, int &anyrot)
+                                          double *Dg, cd *Ja, cd *Jb, int *Fa, int &anyrot)
 {
-    (jac_round<N, LR, Rm1 + 1>(A, U, L, thr, Dg, Rp, Rq, Fa, anyrot), ...);
+    (jac_round<N, LR, Rm1 + 1>(A, U, L, thr, Dg, Ja, Jb, Fa, anyrot), ...);
 }
 
 // Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (rows), eigenvalues
-// in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Rp/Rq[KPW*N], Fa[KPW*N].
+// in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Ja/Jb[KPW*N] (rotation coefficients), Fa[KPW*N].
 template <int N, int LR>
-__device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Rp, cd *Rq, int *Fa,
+__device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Ja, cd *Jb, int *Fa,
                             bool initU = true)
 {
     using G = Geo<N, LR>;
@@ -459,7 +448,7 @@ __device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sw
         s = seg_sum<G::LK>(s);
         const double thr = 1e-15 * (0.5 * s);
         int anyrot = 0;
-        jac_sweep<N, LR>(std::make_integer_sequence<int, N - 1>{}, A, U, L, thr, Dg, Rp, Rq, Fa, anyrot);
+        jac_sweep<N, LR>(std::make_integer_sequence<int, N - 1>{}, A, U, L, thr, Dg, Ja, Jb, Fa, anyrot);
         if (!__any(anyrot)) break;
     }
 }
@@ -492,7 +481,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     __shared__ cd S0[G::KPW * G::NN];
     __shared__ cd S1[G::KPW * G::NN];
     __shared__ double Dg[G::KPW * N];
-    __shared__ cd Rp[G::KPW * N], Rq[G::KPW * N];
+    __shared__ cd Ja[G::KPW * N], Jb[G::KPW * N];
     __shared__ int Fa[G::KPW * N];
 
     const Lane L = lane_id<N, LR>(a.nk, blk);
@@ -537,12 +526,12 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
             store_xrow<N, LR>(s1, L, U);
             wsync();
             lds_product<N, LR, 0, 1>(s0, s1, L, A);  // (U' H0) U'^dagger
-            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa, false);
+            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
         } else {
             phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
             wsync();
             load_xrow<N, LR>(s1, L, A);
-            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
+            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
         }
         store_xrow<N, LR>(s1, L, U);  // U -> S1
         wsync();
@@ -636,7 +625,7 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     __shared__ cd S0[G::KPW * G::NN];
     __shared__ cd S1[G::KPW * G::NN];
     __shared__ double Dg[G::KPW * N];
-    __shared__ cd Rp[G::KPW * N], Rq[G::KPW * N];
+    __shared__ cd Ja[G::KPW * N], Jb[G::KPW * N];
     __shared__ int Fa[G::KPW * N];
     const long blk = blockIdx.x;
     const Lane L = lane_id<N, LR>(a.nk, blk);
@@ -646,7 +635,7 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     wsync();
     cd A[C], U[C];
     load_xrow<N, LR>(s1, L, A);
-    wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Rp, Rq, Fa);
+    wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
     if (a.eig && L.valid && L.h == 0 && L.i < n) a.eig[L.k * n + L.i] = Dg[L.kk * N + L.i];
     if (a.U && L.valid && L.i < n) {
 #pragma unroll
