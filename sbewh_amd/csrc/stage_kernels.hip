@@ -299,9 +299,9 @@ __device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
 
 __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s)
 {
-    double t = y / (fabs(x) + sqrt(x * x + y * y));
-    double cc = 1 / sqrt(1 + t * t);
-    double ss = t * cc;
+    const double t = y / (fabs(x) + sqrt(fma(x, x, y * y)));
+    const double cc = rsqrt(fma(t, t, 1.0));
+    const double ss = t * cc;
     if (x >= 0) {
         c = cc;
         s = ss;
@@ -331,28 +331,40 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
     constexpr int rc = R % C, rh = R / C;
     constexpr int mrow = R * LR + rh;   // lane mask of the row partner (row i ^ R)
     const int base = L.kk * N;
-    // (a) rotation of the pair, computed by the owner of element (p, p^R), p < p^R
-    if (L.h == rh && L.i < (L.i ^ R)) {
-        const int q = L.i ^ R;
-        const cd apq = A[rc];
-        const double app = Dg[base + L.i], aqq = Dg[base + q];
-        const double mod = hypot(apq.x, apq.y);
-        const int act = L.valid && (mod > thr);
-        cd jpp = mk(1, 0), jpq = mk(0, 0), jqq = mk(1, 0), jqp = mk(0, 0);
-        if (act) {
-            double c1, s1, c2, s2;
-            cs_half(-apq.x, apq.y, c1, s1);
-            cs_half(2 * mod, app - aqq, c2, s2);
-            jpp = mk(-s1 * s2, -c1 * s2);
-            jpq = mk(-c1 * c2, -s1 * c2);
-            jqq = mk(-s1 * s2, c1 * s2);   // conj(J_pp)
-            jqp = mk(c1 * c2, -s1 * c2);   // -conj(J_pq)
+    // (a) rotation of the pair (p, q = p ^ R).  Lanes h == rh of rows p and q
+    // hold elements (p, q) and (q, p) = conj(a_pq) in register rc; lane p
+    // computes the phase half-angle of a_pq, lane q the rotation half-angle,
+    // and they swap results (lane ^ R LR).  Both store their row's coefficients.
+    {
+        constexpr int mpair = R * LR;
+        const bool isp = L.i < (L.i ^ R);
+        const cd e = A[rc];
+        // |a_pq| from lane p for both lanes (lane q's conj(a_pq) comes from
+        // other roundings), so that the pair's two lanes agree on act
+        // (every exchange runs in all lanes: a DPP read from a lane that is
+        // switched off returns 0)
+        const double mine = sqrt(fma(e.x, e.x, e.y * e.y));
+        const double other = xchg<mpair>(mk(mine, 0)).x;
+        const double mod = isp ? mine : other;
+        const int pr = isp ? L.i : (L.i ^ R);
+        const double app = Dg[base + pr], aqq = Dg[base + (pr ^ R)];
+        double ch, sh;
+        cs_half(isp ? -e.x : 2 * mod, isp ? e.y : app - aqq, ch, sh);
+        const cd o = xchg<mpair>(mk(ch, sh));
+        const double c1 = isp ? ch : o.x, s1 = isp ? sh : o.y;
+        const double c2 = isp ? o.x : ch, s2 = isp ? o.y : sh;
+        if (L.h == rh) {
+            const int act = L.valid && (mod > thr);
+            cd ja = mk(1, 0), jb = mk(0, 0);
+            if (act) {
+                // row p: (J_pp, J_pq);  row q: (J_qq, J_qp) = (conj(J_pp), -conj(J_pq))
+                ja = mk(-s1 * s2, isp ? -c1 * s2 : c1 * s2);
+                jb = mk(isp ? -c1 * c2 : c1 * c2, -s1 * c2);
+            }
+            Ja[base + L.i] = ja;
+            Jb[base + L.i] = jb;
+            if (isp) Fa[base + L.i] = act;
         }
-        Ja[base + L.i] = jpp;
-        Jb[base + L.i] = jpq;
-        Ja[base + q] = jqq;
-        Jb[base + q] = jqp;
-        Fa[base + L.i] = act;
     }
     wsync();
     const int arow = Fa[base + min(L.i, L.i ^ R)];
