@@ -344,6 +344,10 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         // (every exchange runs in all lanes: a DPP read from a lane that is
         // switched off returns 0)
         const double mine = sqrt(fma(e.x, e.x, e.y * e.y));
+        // a round in which no pair of the wavefront rotates leaves A and U
+        // unchanged: skip it before computing any rotation
+        if (!__any(L.h == rh && isp && L.valid && mine > thr)) return;
+        anyrot = 1;
         const double other = xchg<mpair>(mk(mine, 0)).x;
         const double mod = isp ? mine : other;
         const int pr = isp ? L.i : (L.i ^ R);
@@ -368,12 +372,6 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
     }
     wsync();
     const int arow = Fa[base + min(L.i, L.i ^ R)];
-    anyrot |= arow;
-    if (!__any(arow)) {
-        // no rotation in this round anywhere in the wavefront: A, U unchanged
-        wsync();
-        return;
-    }
     const cd ca = Ja[base + L.i], cb = Jb[base + L.i];
 #pragma unroll
     for (int c = 0; c < C; c++) {
