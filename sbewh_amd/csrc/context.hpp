@@ -155,6 +155,10 @@ struct sbe_ctx {
     long twL_off = 0;
     double *st[10] = {};  // x, k1..k7, scratch, xn
     double *ucache = nullptr;  // [k][n*n] eigenvectors of the last RHS (warm starts)
+    // per-stage phase data of one attempt (twiddles, merged model box, partial
+    // phase sum, contraction scratch), SET_MAX copies each
+    double *tw6 = nullptr, *hm6 = nullptr, *Y6 = nullptr, *y16 = nullptr;
+    long tw_stride = 0, hm_stride = 0, Y_stride = 0, y1_stride = 0;
     bool ucache_valid = false;
     double *err_partial = nullptr;
     unsigned long long *err_host = nullptr;      // pinned, mapped, fine-grained
@@ -273,11 +277,18 @@ struct sbe_ctx {
         HIPCHK(launch_phase_prep(geom, &in, 1, tw, merge_of(E), s));
         HIPCHK(launch_phase_partial(geom, pbufs(), 2, E[0], E[1], E[2], Y, merged(), s));
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
+        stage_dev(in, dxdt, E, last(Y), s, xout);
+    }
+
+    // the stage kernel alone, on a partial phase sum formed beforehand
+    void stage_dev(const StageInput &in, double *dxdt, const double E[3], const PhaseLast &ph, hipStream_t s,
+                   double *xout)
+    {
         StageArgs a{};
         a.in = in;
         a.xout = xout;
         a.dxdt = dxdt;
-        a.ph = last(Y);
+        a.ph = ph;
         a.nk = nk;
         a.n = n;
         a.P = P;
@@ -296,6 +307,52 @@ struct sbe_ctx {
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
         rhs_calls++;
     }
+
+    // The phase sums of the m stage inputs of one attempt (times of E[q]), all
+    // formed before the first stage runs: they depend on rho only through the
+    // special slots A, and the A slot of a derivative computed in this attempt
+    // is -E at its stage time, known here (made[j] = stage whose output k_j is,
+    // or -1 for a derivative already in memory).
+    void phase_set(const StageInput *ins, const double (*E)[3], const double *const *outs, int m, hipStream_t s)
+    {
+        PrepSet ps{};
+        ps.m = m;
+        ContractSet cs[2]{};
+        int ns = 0;
+        for (int q = 0; q < m; q++) {
+            TwiddleEntry &e = ps.e[q];
+            e.g = geom;
+            e.in = ins[q];
+            e.has_a = 1;
+            e.use_shift = 1;
+            e.tw = tw6 + q * tw_stride;
+            e.mm = merge_of(E[q]);
+            e.mm.out = dim == 1 ? Y6 + q * Y_stride : hm6 + q * hm_stride;
+            AOverride &ao = ps.a[q];
+            for (int j = 0; j < ins[q].nt; j++)
+                for (int r = 0; r < q; r++)
+                    if (ins[q].k[j] == outs[r]) {
+                        ao.mask |= 1 << j;
+                        for (int d = 0; d < 3; d++) ao.v[j][d] = d < dim ? -E[r][d] : 0.0;
+                    }
+            ps.max_count = std::max(ps.max_count, (int)(twiddle_offset(geom, 3) + (e.mm.out ? e.mm.nbox * 2 * nn : 0)));
+            PhaseBuffers pb{Hb, Db, dHb, kmap, tw6 + q * tw_stride, y16 + q * y1_stride, nullptr};
+            ContractStage st[2];
+            ns = phase_partial_plan(geom, pb, 2, E[q][0], E[q][1], E[q][2], Y6 + q * Y_stride, e.mm.out, st);
+            for (int d = 0; d < ns; d++) {
+                cs[d].st[q] = st[d];
+                cs[d].block0[q + 1] = cs[d].block0[q] + st[d].blocks_x * st[d].blocks_y;
+                cs[d].m = m;
+            }
+        }
+        HIPCHK(launch_prep_set(ps, s));
+        for (int d = 0; d < ns; d++) HIPCHK(launch_contract_set(cs[d], s));
+    }
+    PhaseLast last_set(int q) const
+    {
+        return PhaseLast{Y6 + q * Y_stride, tw6 + q * tw_stride + 2 * twL_off, kphase, WL};
+    }
+
     void rhs_dev(const double *x_in, double *dxdt, double t, hipStream_t s)
     {
         StageInput in{};
@@ -338,12 +395,27 @@ struct sbe_ctx {
             for (double c : cs) in.c[j++] = dt * c;
             return in;
         };
-        rhs_dev(input({k1}, {B21}), k2, t + dt * A2, s);
-        rhs_dev(input({k1, k2}, {B31, B32}), k3, t + dt * A3, s);
-        rhs_dev(input({k1, k2, k3}, {B41, B42, B43}), k4, t + dt * A4, s);
-        rhs_dev(input({k1, k2, k3, k4}, {B51, B52, B53, B54}), k5, t + dt * A5, s);
-        rhs_dev(input({k1, k2, k3, k4, k5}, {B61, B62, B63, B64, B65}), k6, t + dt, s);
-        rhs_dev(input({k1, k3, k4, k5, k6}, {C1, C3, C4, C5, C6}), k7, t + dt, s, xn);
+        const StageInput ins[6] = {input({k1}, {B21}),
+                                   input({k1, k2}, {B31, B32}),
+                                   input({k1, k2, k3}, {B41, B42, B43}),
+                                   input({k1, k2, k3, k4}, {B51, B52, B53, B54}),
+                                   input({k1, k2, k3, k4, k5}, {B61, B62, B63, B64, B65}),
+                                   input({k1, k3, k4, k5, k6}, {C1, C3, C4, C5, C6})};
+        double *const outs[6] = {k2, k3, k4, k5, k6, k7};
+        const double ts[6] = {t + dt * A2, t + dt * A3, t + dt * A4, t + dt * A5, t + dt, t + dt};
+        double E[6][3];
+        for (int q = 0; q < 6; q++) E_at(ts[q], E[q]);
+        // the six phase sums first, then the six stage kernels
+        if (timing) HIPCHK(hipEventRecord(next_event(), s));
+        phase_set(ins, E, outs, 6, s);
+        if (timing) HIPCHK(hipEventRecord(next_event(), s));
+        for (int q = 0; q < 6; q++) {
+            if (timing && q > 0) {
+                HIPCHK(hipEventRecord(next_event(), s));
+                HIPCHK(hipEventRecord(next_event(), s));
+            }
+            stage_dev(ins[q], outs[q], E[q], last_set(q), s, q == 5 ? xn : nullptr);
+        }
         const double *ek[6] = {k1, k3, k4, k5, k6, k7};
         double e[6] = {dt * DC1, dt * DC3, dt * DC4, dt * DC5, dt * DC6, dt * DC7};
         *reinterpret_cast<volatile unsigned long long *>(err_host) = ERR_PENDING;
@@ -385,7 +457,7 @@ struct sbe_ctx {
 
     ~sbe_ctx()
     {
-        double *ptrs[] = {Hb, Db, dHb, tw, y1, y2, mesh, Y, hm, Yobs, partial, jbuf, ucache};
+        double *ptrs[] = {Hb, Db, dHb, tw, y1, y2, mesh, Y, hm, Yobs, partial, jbuf, ucache, tw6, hm6, Y6, y16};
         for (double *p : ptrs)
             if (p) (void)hipFree(p);
         for (double *p : st)
@@ -620,6 +692,16 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
         HIPCHK(hipMemset(p, 0, 16 * c->S));
     }
     c->ucache = dalloc<double>(2 * c->nk * nn);
+    if (!member_of_scan) {
+        c->tw_stride = 2 * ntw;
+        c->hm_stride = g->dim > 1 ? 2 * nbox * 2 * nn : 0;
+        c->Y_stride = 2 * phase_partial_size(G, 2);
+        c->y1_stride = 2 * phase_partial_scratch(G, 2);
+        c->tw6 = dalloc<double>(SET_MAX * c->tw_stride);
+        if (g->dim > 1) c->hm6 = dalloc<double>(SET_MAX * c->hm_stride);
+        c->Y6 = dalloc<double>(SET_MAX * c->Y_stride);
+        c->y16 = dalloc<double>(SET_MAX * c->y1_stride);
+    }
     c->err_partial = dalloc<double>(1024);
     if (!member_of_scan) {
         HIPCHK(hipHostMalloc((void **)&c->err_host, sizeof(unsigned long long),

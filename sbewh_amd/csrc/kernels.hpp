@@ -126,6 +126,32 @@ struct TwiddleEntry {
 };
 // grid: ceil(max_count / 256) x m, max_count = max_j (sum_d Nk_d W_d + merged entries)
 hipError_t launch_twiddles_batched(const TwiddleEntry *d_entries, int m, int max_count, hipStream_t s);
+
+// ---- the phase sums of the RHS evaluations of one Dormand-Prince attempt,
+// formed before its first stage runs: one launch for every twiddle set and
+// merged model box, one per contraction pass.  The entries travel as kernel
+// arguments (no upload).
+constexpr int SET_MAX = 6;
+// Special-slot (A) values of stage derivatives that a stage input combines but
+// that are not computed when the set is formed: slot e of derivative j of the
+// input is v[j][e] where bit j of mask is set (the value the stage kernel will
+// store there, -E_e(t_j)), else it is read from memory.
+struct AOverride {
+    double v[6][3];
+    int mask;
+};
+struct PrepSet {
+    TwiddleEntry e[SET_MAX];
+    AOverride a[SET_MAX];
+    int m, max_count;
+};
+hipError_t launch_prep_set(const PrepSet &ps, hipStream_t s);
+struct ContractSet {
+    ContractStage st[SET_MAX];
+    long block0[SET_MAX + 1];  // entry j owns blocks [block0[j], block0[j+1])
+    int m;
+};
+hipError_t launch_contract_set(const ContractSet &cs, hipStream_t s);
 // entry j owns blocks [block0[j], block0[j] + blocks_x * blocks_y); blocks = total
 hipError_t launch_contract_batched(const ContractStage *d_stages, const long *d_block0, int m, long blocks,
                                    hipStream_t s);

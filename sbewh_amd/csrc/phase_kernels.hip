@@ -22,7 +22,7 @@
 namespace sbe {
 
 __device__ __forceinline__ void twiddle_body(const PhaseGeom &g, const StageInput &in, int has_a, int use_shift,
-                                             double *tw, int t)
+                                             double *tw, int t, const AOverride *ao = nullptr)
 {
     int n0 = g.Nk[0] * g.W[0], n1 = g.Nk[1] * g.W[1], n2 = g.Nk[2] * g.W[2];
     if (t >= n0 + n1 + n2) return;
@@ -37,7 +37,7 @@ __device__ __forceinline__ void twiddle_body(const PhaseGeom &g, const StageInpu
             A = in.x[2 * e];
 #pragma unroll
             for (int j = 0; j < 6; j++)
-                if (j < in.nt) A += in.c[j] * in.k[j][2 * e];
+                if (j < in.nt) A += in.c[j] * ((ao && ((ao->mask >> j) & 1)) ? ao->v[j][e] : in.k[j][2 * e]);
         }
         k_au[e] = (use_shift ? g.kshift_au[e] : 0.0) + A;
     }
@@ -84,6 +84,17 @@ __device__ __forceinline__ void prep_body(const PhaseGeom &g, const StageInput &
 __global__ void twiddle_kernel(PhaseGeom g, StageInput in, int has_a, int use_shift, double *tw, ModelMerge mm)
 {
     prep_body(g, in, has_a, use_shift, tw, mm, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// entry blockIdx.y of the set
+__global__ void prep_set_kernel(PrepSet ps)
+{
+    const int j = blockIdx.y;
+    const TwiddleEntry &q = ps.e[j];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ntw = twiddle_count(q.g);
+    if (t < ntw) twiddle_body(q.g, q.in, q.has_a, q.use_shift, q.tw, (int)t, &ps.a[j]);
+    else merge_body(q.mm, t - ntw);
 }
 
 // one instance per blockIdx.y
@@ -244,6 +255,15 @@ __device__ __forceinline__ void stage_body(const ContractStage &st, long lb)
 __global__ __launch_bounds__(256) void contract_stage_kernel(ContractStage st)
 {
     stage_body(st, (long)blockIdx.y * st.blocks_x + blockIdx.x);
+}
+
+// entry j of the set owns the blocks [block0[j], block0[j+1])
+__global__ __launch_bounds__(256) void contract_set_kernel(ContractSet cs)
+{
+    const long gb = blockIdx.x;
+    int j = 0;
+    while (j + 1 < cs.m && cs.block0[j + 1] <= gb) j++;
+    stage_body(cs.st[j], gb - cs.block0[j]);
 }
 
 // entry j owns the blocks [block0[j], block0[j+1])
@@ -437,6 +457,25 @@ hipError_t launch_phase_partial(const PhaseGeom &g, const PhaseBuffers &b, int o
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_prep_set(const PrepSet &ps, hipStream_t s)
+{
+    if (ps.m <= 0) return hipSuccess;
+    if (ps.m > SET_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(prep_set_kernel, dim3((unsigned)((ps.max_count + 255) / 256), (unsigned)ps.m), dim3(256), 0, s,
+                       ps);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_set(const ContractSet &cs, hipStream_t s)
+{
+    if (cs.m <= 0) return hipSuccess;
+    if (cs.m > SET_MAX) return hipErrorInvalidValue;
+    const long blocks = cs.block0[cs.m];
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(contract_set_kernel, dim3((unsigned)blocks), dim3(256), 0, s, cs);
+    return hipGetLastError();
 }
 
 hipError_t launch_twiddles_batched(const TwiddleEntry *d_entries, int m, int max_count, hipStream_t s)
