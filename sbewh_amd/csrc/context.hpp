@@ -159,6 +159,7 @@ struct sbe_ctx {
     // phase sum, contraction scratch), SET_MAX copies each
     double *tw6 = nullptr, *hm6 = nullptr, *Y6 = nullptr, *y16 = nullptr;
     long tw_stride = 0, hm_stride = 0, Y_stride = 0, y1_stride = 0;
+    long set_min_blocks = 512;  // block target of a set's contraction launch (measured best at c2)
     bool ucache_valid = false;
     double *err_partial = nullptr;
     unsigned long long *err_host = nullptr;      // pinned, mapped, fine-grained
@@ -338,7 +339,9 @@ struct sbe_ctx {
             ps.max_count = std::max(ps.max_count, (int)(twiddle_offset(geom, 3) + (e.mm.out ? e.mm.nbox * 2 * nn : 0)));
             PhaseBuffers pb{Hb, Db, dHb, kmap, tw6 + q * tw_stride, y16 + q * y1_stride, nullptr};
             ContractStage st[2];
-            ns = phase_partial_plan(geom, pb, 2, E[q][0], E[q][1], E[q][2], Y6 + q * Y_stride, e.mm.out, st);
+            // the set's stages share one launch: larger row tiles reuse the inputs
+            ns = phase_partial_plan(geom, pb, 2, E[q][0], E[q][1], E[q][2], Y6 + q * Y_stride, e.mm.out, st,
+                                    set_min_blocks / m);
             for (int d = 0; d < ns; d++) {
                 cs[d].st[q] = st[d];
                 cs[d].block0[q + 1] = cs[d].block0[q] + st[d].blocks_x * st[d].blocks_y;

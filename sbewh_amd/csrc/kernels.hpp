@@ -96,9 +96,10 @@ hipError_t launch_phase_sum(const PhaseGeom &g, const PhaseBuffers &b, int ops, 
 // Y itself and no stage runs), or null to read the model operators directly.
 hipError_t launch_phase_partial(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1,
                                 double E2, double *Y, const double *merged, hipStream_t s);
-// the stages of launch_phase_partial (at most 2), returns their count
+// the stages of launch_phase_partial (at most 2), returns their count; a tiled
+// stage gets the largest row tile that leaves >= min_blocks blocks
 int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1, double E2,
-                       double *Y, const double *merged, ContractStage *st);
+                       double *Y, const double *merged, ContractStage *st, long min_blocks = 2048);
 // complex entries of Y and of the stage scratch it needs (y1)
 long phase_partial_size(const PhaseGeom &g, int ops);
 long phase_partial_scratch(const PhaseGeom &g, int ops);

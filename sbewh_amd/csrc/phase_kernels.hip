@@ -301,7 +301,7 @@ long twiddle_offset(const PhaseGeom &g, int d)
 
 // grid of one contraction stage: the tiled kernel when the R extent fits its
 // register tile, the generic one otherwise
-static ContractStage plan_contract(const ContractArgs &a0)
+static ContractStage plan_contract(const ContractArgs &a0, long min_blocks = 2048)
 {
     ContractStage st{};
     st.a = a0;
@@ -309,10 +309,10 @@ static ContractStage plan_contract(const ContractArgs &a0)
     if (a.W <= WMAX) {
         long ncol = a.outer * a.inner * a.M;
         long cb = (ncol + 255) / 256;
-        // smallest i-tile that still gives >= 2048 blocks (fills 256 CUs),
-        // larger tiles reuse the inputs held in registers more often
+        // largest i-tile (inputs held in registers are reused ti times) that
+        // still gives >= min_blocks blocks (2048 fill 256 CUs)
         int ti = TI;
-        while (ti > 1 && cb * ((a.Nkd + ti - 1) / ti) < 2048) ti >>= 1;
+        while (ti > 1 && cb * ((a.Nkd + ti - 1) / ti) < min_blocks) ti >>= 1;
         a.ti = ti;
         st.kind = 0;
         st.blocks_x = cb;
@@ -399,7 +399,7 @@ long phase_partial_scratch(const PhaseGeom &g, int ops)
 }
 
 int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, double E0, double E1, double E2,
-                       double *Y, const double *merged, ContractStage *st)
+                       double *Y, const double *merged, ContractStage *st, long min_blocks)
 {
     const int nn = g.n * g.n;
     ContractArgs a{};
@@ -440,7 +440,7 @@ int phase_partial_plan(const PhaseGeom &g, const PhaseBuffers &b, int ops, doubl
         a.tw = b.tw + 2 * twiddle_offset(g, d);
         a.out_mode = 0;
         a.out = (d == L - 1) ? Y : b.y1;
-        st[d] = plan_contract(a);
+        st[d] = plan_contract(a, min_blocks);
         src = a.out;
     }
     return L;
