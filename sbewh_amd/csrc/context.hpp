@@ -283,9 +283,17 @@ struct sbe_ctx {
 
     // the stage kernel alone, on a partial phase sum formed beforehand
     void stage_dev(const StageInput &in, double *dxdt, const double E[3], const PhaseLast &ph, hipStream_t s,
-                   double *xout)
+                   double *xout, const double *err_e = nullptr, double dt = 0)
     {
         StageArgs a{};
+        if (err_e) {
+            // the attempt's error norm, per block, fused into this (last) stage
+            a.err_partial = err_partial;
+            for (int j = 0; j < 6; j++) a.err_e[j] = err_e[j];
+            a.err_ea = prm.eps_abs;
+            a.err_er = prm.eps_rel;
+            a.err_dt = dt;
+        }
         a.in = in;
         a.xout = xout;
         a.dxdt = dxdt;
@@ -408,6 +416,8 @@ struct sbe_ctx {
         const double ts[6] = {t + dt * A2, t + dt * A3, t + dt * A4, t + dt * A5, t + dt, t + dt};
         double E[6][3];
         for (int q = 0; q < 6; q++) E_at(ts[q], E[q]);
+        // error coefficients of k1, k3, k4, k5, k6, k7
+        const double err_e[6] = {dt * DC1, dt * DC3, dt * DC4, dt * DC5, dt * DC6, dt * DC7};
         // the six phase sums first, then the six stage kernels
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
         phase_set(ins, E, outs, 6, s);
@@ -417,12 +427,12 @@ struct sbe_ctx {
                 HIPCHK(hipEventRecord(next_event(), s));
                 HIPCHK(hipEventRecord(next_event(), s));
             }
-            stage_dev(ins[q], outs[q], E[q], last_set(q), s, q == 5 ? xn : nullptr);
+            stage_dev(ins[q], outs[q], E[q], last_set(q), s, q == 5 ? xn : nullptr, q == 5 ? err_e : nullptr, dt);
         }
-        const double *ek[6] = {k1, k3, k4, k5, k6, k7};
-        double e[6] = {dt * DC1, dt * DC3, dt * DC4, dt * DC5, dt * DC6, dt * DC7};
+        // the last stage kernel left one error-norm partial per block
         *reinterpret_cast<volatile unsigned long long *>(err_host) = ERR_PENDING;
-        HIPCHK(launch_error_norm(S, X, ek, e, prm.eps_abs, prm.eps_rel, dt, err_partial, err_host_dev, s));
+        HIPCHK(launch_error_final(err_partial, (nk + stage_k_per_block(N, lr) - 1) / stage_k_per_block(N, lr),
+                                  err_host_dev, s));
         return wait_error(s);
     }
 
@@ -705,7 +715,7 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
         c->Y6 = dalloc<double>(SET_MAX * c->Y_stride);
         c->y16 = dalloc<double>(SET_MAX * c->y1_stride);
     }
-    c->err_partial = dalloc<double>(1024);
+    c->err_partial = dalloc<double>(std::max(1024L, c->nk));  // one per stage block (>= 1 k-point each)
     if (!member_of_scan) {
         HIPCHK(hipHostMalloc((void **)&c->err_host, sizeof(unsigned long long),
                              hipHostMallocMapped | hipHostMallocCoherent));

@@ -30,6 +30,25 @@ __device__ __forceinline__ cd cfma_jc(cd a, cd b, cd acc)
 {
     return cd{fma(a.x, b.x, fma(a.y, b.y, acc.x)), fma(-a.x, b.y, fma(a.y, b.x, acc.y))};
 }
+// Relative error term of one state entry of a Dormand-Prince attempt,
+//   |sum_j e[j] k_j| / (ea + er (|x| + dt |k_0|)),  k_0..k_5 = k1, k3, k4, k5, k6, k7
+// (the controller's error checker, code/src/main/runSBE.cpp:178-180 via
+// Boost.Odeint), with a fixed operation order: every kernel that evaluates
+// it gets the same bits.
+__device__ __forceinline__ double err_entry(double2 xv, const double2 (&kv)[6], const double (&e)[6], double ea,
+                                            double er, double dt)
+{
+    double re = 0, im = 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        re = fma(e[j], kv[j].x, re);
+        im = fma(e[j], kv[j].y, im);
+    }
+    const double num = sqrt(fma(re, re, im * im));
+    const double ax = sqrt(fma(xv.x, xv.x, xv.y * xv.y)), ak = sqrt(fma(kv[0].x, kv[0].x, kv[0].y * kv[0].y));
+    return num / fma(er, fma(dt, ak, ax), ea);
+}
+
 // Last-axis contraction of a partial phase sum (see PhaseLast in kernels.hpp)
 // for k-point k, entry e of an M-wide operator row.
 template <class PL>

@@ -174,6 +174,13 @@ struct StageArgs {
     int dephase;           // F2 > 0
     double *ucache;        // [k][n*n] eigenvectors of the previous call (or null)
     int warm;              // 1: start the Jacobi from ucache instead of the identity
+    // error norm of the attempt fused into its last stage: if err_partial is
+    // set, in.k[0..4] are k1, k3, k4, k5, k6 and dxdt is k7; block b stores
+    // max over its entries (block 0: and the special slots) of err_entry with
+    // coefficients err_e, err_ea, err_er, err_dt into err_partial[b]
+    double *err_partial;
+    double err_e[6];
+    double err_ea, err_er, err_dt;
 };
 // N: padded band count (2, 4, 8, 16); LR: lanes per matrix row (layout variant)
 hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s);
@@ -201,11 +208,12 @@ struct InitArgs {
 hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s);
 
 // ---- RK stage algebra (rk_kernels.hip) -----------------------------------
-// err_max = max_i |sum_j e[j] k[j]_i| / (ea + er (|x_i| + dt |k0_i|)), stored as the bit
-// pattern of a non-negative double into *out (device memory or the device alias of
-// mapped host memory) with a system-scope store; partial holds >= 1024 doubles.
-hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
-                             double er, double dt, double *partial, unsigned long long *out, hipStream_t s);
+// err_max = max_i |sum_j e[j] k[j]_i| / (ea + er (|x_i| + dt |k0_i|)) (err_entry):
+// a single instance's last stage kernel leaves per-block maxima (StageArgs::
+// err_partial); this reduces them and stores the bit pattern of the
+// non-negative max into *out (device memory or the device alias of mapped
+// host memory) with a system-scope store.
+hipError_t launch_error_final(const double *partial, long nb, unsigned long long *out, hipStream_t s);
 
 // per-instance error norms: entry j covers state S_j of its instance and
 // writes its norm to *err_out (device); partial has >= sum_j blocks_j doubles

@@ -1,6 +1,7 @@
-// Dormand-Prince error norm over the whole state (FP64, HBM-bound).  The
-// stage inputs x + (dt b_1) k_1 + (dt b_2) k_2 + ... are formed by the stage
-// kernel that consumes them.
+// Dormand-Prince error norm (FP64).  The stage inputs x + (dt b_1) k_1 + ...
+// are formed by the stage kernel that consumes them, and a single instance's
+// error terms by its last stage kernel (per-block maxima, reduced here); the
+// batched form below covers the instances of a scan.
 //
 // Boost.Odeint's runge_kutta_dopri5 forms the error estimate as
 // sum_j (dt dc_j) k_j, normalised by
@@ -13,33 +14,14 @@
 
 namespace sbe {
 
-struct ErrArgs {
-    const double *x;
-    const double *k[6];   // k1 (= dxdt at the step start), k3, k4, k5, k6, k7
-    double e[6];
-    double ea, er, dt;
-    long S;
-    double *partial;      // [ERR_BLOCKS]
-};
-
-constexpr int ERR_BLOCKS = 1024;
-
 // relative error of state entry i
 template <class A>
 __device__ __forceinline__ double err_term(const A &a, long i)
 {
-    double re = 0, im = 0;
+    double2 kv[6];
 #pragma unroll
-    for (int j = 0; j < 6; j++) {
-        double2 kv = reinterpret_cast<const double2 *>(a.k[j])[i];
-        re += a.e[j] * kv.x;
-        im += a.e[j] * kv.y;
-    }
-    double2 xv = reinterpret_cast<const double2 *>(a.x)[i];
-    double2 d1 = reinterpret_cast<const double2 *>(a.k[0])[i];
-    double num = sqrt(re * re + im * im);
-    double den = a.ea + a.er * (sqrt(xv.x * xv.x + xv.y * xv.y) + a.dt * sqrt(d1.x * d1.x + d1.y * d1.y));
-    return num / den;
+    for (int j = 0; j < 6; j++) kv[j] = reinterpret_cast<const double2 *>(a.k[j])[i];
+    return err_entry(reinterpret_cast<const double2 *>(a.x)[i], kv, a.e, a.ea, a.er, a.dt);
 }
 
 // max over the 256 threads of a block (result valid in thread 0)
@@ -52,19 +34,10 @@ __device__ __forceinline__ double block_max(double v)
     return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// pass 1: grid-stride max over the state, one partial per block
-__global__ __launch_bounds__(256) void error_kernel(ErrArgs a)
-{
-    double v = 0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.S; i += (long)gridDim.x * blockDim.x)
-        v = fmax(v, err_term(a, i));
-    v = block_max(v);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = v;
-}
-
-// pass 2: one block reduces the partials (the kernel boundary orders them)
-// and stores the bit pattern of the max with a system-scope store, so that a
-// host polling mapped memory sees it without waiting for the stream to drain
+// Final reduction of per-block maxima (the last stage kernel of an attempt
+// leaves one per block; the kernel boundary orders them): stores the bit
+// pattern of the max with a system-scope store, so that a host polling mapped
+// memory sees it without waiting for the stream to drain
 __global__ __launch_bounds__(256) void error_final_kernel(const double *partial, int nb, unsigned long long *out)
 {
     double v = 0;
@@ -102,27 +75,10 @@ __global__ __launch_bounds__(256) void error_batched_final_kernel(const ErrEntry
     if (threadIdx.x == 0) *e[j].err_out = v;
 }
 
-hipError_t launch_error_norm(long S, const double *x, const double *const *k, const double *e, double ea,
-                             double er, double dt, double *partial, unsigned long long *out, hipStream_t s)
+hipError_t launch_error_final(const double *partial, long nb, unsigned long long *out, hipStream_t s)
 {
-    ErrArgs a{};
-    a.x = x;
-    for (int j = 0; j < 6; j++) {
-        a.k[j] = k[j];
-        a.e[j] = e[j];
-    }
-    a.ea = ea;
-    a.er = er;
-    a.dt = dt;
-    a.S = S;
-    a.partial = partial;
-    long need = (S + 255) / 256;
-    int nb = (int)(need < ERR_BLOCKS ? need : ERR_BLOCKS);
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(error_kernel, dim3(nb), dim3(256), 0, s, a);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
+    if (nb < 1 || nb > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, partial, (int)nb, out);
     return hipGetLastError();
 }
 

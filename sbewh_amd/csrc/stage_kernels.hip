@@ -482,6 +482,46 @@ __device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, 
     }
 }
 
+// Error norm of the attempt over this block's k-points (last stage only): the
+// stage input's x, k1, k3..k6 are read again (HBM that the VALU-bound kernel
+// leaves idle), k7 = d rho/dt comes from LDS natural layout; same lane ->
+// entry mapping as load_rho, two entries' loads in flight at a time.
+template <int N, int LR>
+__device__ __forceinline__ double stage_err(const StageArgs &a, const cd *lds, long blk)
+{
+    using G = Geo<N, LR>;
+    const int n = a.n;
+    const long k0 = blk * G::KPW;
+    const double2 *x2 = reinterpret_cast<const double2 *>(a.in.x);
+    double v = 0;
+    constexpr int IT = G::KPW * G::NN / 64;
+#pragma unroll 2
+    for (int u = 0; u < IT; u++) {
+        const int e = threadIdx.x + 64 * u;
+        const int kk = e / G::NN, idx = e % G::NN, r = idx / N, c = idx % N;
+        const long k = k0 + kk;
+        if (r < n && c < n && k < a.nk) {
+            const long i = 8 + k * a.P + r * n + c;
+            double2 kv[6];
+#pragma unroll
+            for (int j = 0; j < 5; j++) kv[j] = reinterpret_cast<const double2 *>(a.in.k[j])[i];
+            const cd d = lds[kk * G::NN + sw<N>(r, c)];
+            kv[5] = make_double2(d.x, d.y);
+            v = fmax(v, err_entry(x2[i], kv, a.err_e, a.err_ea, a.err_er, a.err_dt));
+        }
+    }
+    if (blk == 0 && threadIdx.x < 8) {
+        // special slots: d/dt of A_t is -E_t (t < dim), the rest 0
+        const int t = threadIdx.x;
+        double2 kv[6];
+#pragma unroll
+        for (int j = 0; j < 5; j++) kv[j] = reinterpret_cast<const double2 *>(a.in.k[j])[t];
+        kv[5] = make_double2(t < a.dim ? -a.E[t] : 0.0, 0.0);
+        v = fmax(v, err_entry(x2[t], kv, a.err_e, a.err_ea, a.err_er, a.err_dt));
+    }
+    return v;
+}
+
 // the stage for the KPW k-points of logical block blk of one instance
 template <int N, int LR>
 __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
@@ -606,6 +646,10 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     store_xrow<N, LR>(s0, L, dr);
     wsync();
     store_state<N, LR>(a.dxdt, S0, n, a.P, a.nk, blk);
+    if (a.err_partial) {
+        const double v = seg_max<64>(stage_err<N, LR>(a, S0, blk));
+        if (threadIdx.x == 0) a.err_partial[blk] = v;
+    }
 }
 
 template <int N, int LR>
