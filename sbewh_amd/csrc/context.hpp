@@ -324,10 +324,10 @@ struct sbe_ctx {
     // or -1 for a derivative already in memory).
     void phase_set(const StageInput *ins, const double (*E)[3], const double *const *outs, int m, hipStream_t s)
     {
+        // the prep launch goes out before the contraction is planned: the GPU
+        // is idle from the error norm's arrival until it starts
         PrepSet ps{};
         ps.m = m;
-        ContractSet cs[2]{};
-        int ns = 0;
         for (int q = 0; q < m; q++) {
             TwiddleEntry &e = ps.e[q];
             e.g = geom;
@@ -345,10 +345,15 @@ struct sbe_ctx {
                         for (int d = 0; d < 3; d++) ao.v[j][d] = d < dim ? -E[r][d] : 0.0;
                     }
             ps.max_count = std::max(ps.max_count, (int)(twiddle_offset(geom, 3) + (e.mm.out ? e.mm.nbox * 2 * nn : 0)));
+        }
+        HIPCHK(launch_prep_set(ps, s));
+        ContractSet cs[2]{};
+        int ns = 0;
+        for (int q = 0; q < m; q++) {
             PhaseBuffers pb{Hb, Db, dHb, kmap, tw6 + q * tw_stride, y16 + q * y1_stride, nullptr};
             ContractStage st[2];
             // the set's stages share one launch: larger row tiles reuse the inputs
-            ns = phase_partial_plan(geom, pb, 2, E[q][0], E[q][1], E[q][2], Y6 + q * Y_stride, e.mm.out, st,
+            ns = phase_partial_plan(geom, pb, 2, E[q][0], E[q][1], E[q][2], Y6 + q * Y_stride, ps.e[q].mm.out, st,
                                     set_min_blocks / m);
             for (int d = 0; d < ns; d++) {
                 cs[d].st[q] = st[d];
@@ -356,7 +361,6 @@ struct sbe_ctx {
                 cs[d].m = m;
             }
         }
-        HIPCHK(launch_prep_set(ps, s));
         for (int d = 0; d < ns; d++) HIPCHK(launch_contract_set(cs[d], s));
     }
     PhaseLast last_set(int q) const
