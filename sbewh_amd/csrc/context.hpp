@@ -159,6 +159,7 @@ struct sbe_ctx {
     // phase sum, contraction scratch), SET_MAX copies each
     double *tw6 = nullptr, *hm6 = nullptr, *Y6 = nullptr, *y16 = nullptr;
     long tw_stride = 0, hm_stride = 0, Y_stride = 0, y1_stride = 0;
+    long fuse_below_blocks = 512;  // stage grids smaller than this run an attempt's stages in one launch
     long set_min_blocks = 512;  // block target of a set's contraction launch (measured best at c2)
     bool ucache_valid = false;
     double *err_partial = nullptr;
@@ -285,6 +286,18 @@ struct sbe_ctx {
     void stage_dev(const StageInput &in, double *dxdt, const double E[3], const PhaseLast &ph, hipStream_t s,
                    double *xout, const double *err_e = nullptr, double dt = 0)
     {
+        StageArgs a = stage_args(in, dxdt, E, ph, xout, err_e, dt);
+        if (!lr) tune_stage(a, s);
+        a.warm = (ucache_valid && !(prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
+        HIPCHK(launch_rhs_stage(N, lr, a, s));
+        if (a.dephase) ucache_valid = true;
+        if (timing) HIPCHK(hipEventRecord(next_event(), s));
+        rhs_calls++;
+    }
+
+    StageArgs stage_args(const StageInput &in, double *dxdt, const double E[3], const PhaseLast &ph, double *xout,
+                         const double *err_e, double dt) const
+    {
         StageArgs a{};
         if (err_e) {
             // the attempt's error norm, per block, fused into this (last) stage
@@ -309,12 +322,7 @@ struct sbe_ctx {
         a.dephase = prm.relaxation_F2 > 0;
         a.ucache = ucache;
         a.warm = 0;
-        if (!lr) tune_stage(a, s);
-        a.warm = (ucache_valid && !(prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
-        HIPCHK(launch_rhs_stage(N, lr, a, s));
-        if (a.dephase) ucache_valid = true;
-        if (timing) HIPCHK(hipEventRecord(next_event(), s));
-        rhs_calls++;
+        return a;
     }
 
     // The phase sums of the m stage inputs of one attempt (times of E[q]), all
@@ -426,12 +434,27 @@ struct sbe_ctx {
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
         phase_set(ins, E, outs, 6, s);
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
-        for (int q = 0; q < 6; q++) {
-            if (timing && q > 0) {
-                HIPCHK(hipEventRecord(next_event(), s));
-                HIPCHK(hipEventRecord(next_event(), s));
+        if (lr && !timing && (nk + stage_k_per_block(N, lr) - 1) / stage_k_per_block(N, lr) < fuse_below_blocks) {
+            // a grid this small is launch-bound: the six stages in one launch
+            StageSet ss{};
+            ss.m = 6;
+            for (int q = 0; q < 6; q++) {
+                ss.st[q] = stage_args(ins[q], outs[q], E[q], last_set(q), q == 5 ? xn : nullptr,
+                                      q == 5 ? err_e : nullptr, dt);
+                ss.st[q].warm = (ucache_valid && !(prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
+                if (ss.st[q].dephase) ucache_valid = true;
             }
-            stage_dev(ins[q], outs[q], E[q], last_set(q), s, q == 5 ? xn : nullptr, q == 5 ? err_e : nullptr, dt);
+            HIPCHK(launch_rhs_stage_set(N, lr, ss, s));
+            rhs_calls += 6;
+        } else {
+            for (int q = 0; q < 6; q++) {
+                if (timing && q > 0) {
+                    HIPCHK(hipEventRecord(next_event(), s));
+                    HIPCHK(hipEventRecord(next_event(), s));
+                }
+                stage_dev(ins[q], outs[q], E[q], last_set(q), s, q == 5 ? xn : nullptr, q == 5 ? err_e : nullptr,
+                          dt);
+            }
         }
         // the last stage kernel left one error-norm partial per block
         *reinterpret_cast<volatile unsigned long long *>(err_host) = ERR_PENDING;

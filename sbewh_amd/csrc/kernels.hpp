@@ -184,6 +184,17 @@ struct StageArgs {
 };
 // N: padded band count (2, 4, 8, 16); LR: lanes per matrix row (layout variant)
 hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s);
+// The stages of one Dormand-Prince attempt in one launch, for grids too small
+// to fill the GPU (launch gaps dominate there): block b runs st[0 .. m) in
+// order on its own k-points.  Stage q reads only what the earlier stages of
+// the same block wrote (its k-points' derivatives and eigenvector cache; in
+// block 0 also the special slots); phase sums and A slots are formed before
+// the launch, so no grid-wide barrier is needed.
+struct StageSet {
+    StageArgs st[SET_MAX];
+    int m;
+};
+hipError_t launch_rhs_stage_set(int N, int LR, const StageSet &ss, hipStream_t s);
 // LR values compiled for N (returns the count written to lr_out)
 int stage_variants(int N, int *lr_out, int cap);
 // k-points per block (one wavefront) of layout (N, LR)

@@ -746,6 +746,33 @@ static hipError_t launch_rhs_NL(const StageArgs &a, hipStream_t s)
 }
 
 template <int N, int LR>
+__global__ __launch_bounds__(64) void rhs_stage_set_kernel(
+    StageSet ss)
+{
+    const long blk = xcd_block(blockIdx.x, gridDim.x);
+    for (int q = 0; q < ss.m; q++) {
+        rhs_stage_body<N, LR>(ss.st[q], blk);
+        // the next stage loads, from other lanes, what this one stored
+        __threadfence_block();
+        wsync();
+    }
+}
+
+template <int N, int LR>
+static hipError_t launch_rhs_set_NL(const StageSet &ss, hipStream_t s)
+{
+    using G = Geo<N, LR>;
+    if (ss.m <= 0) return hipSuccess;
+    if (ss.m > SET_MAX) return hipErrorInvalidValue;
+    for (int q = 1; q < ss.m; q++)
+        if (ss.st[q].nk != ss.st[0].nk) return hipErrorInvalidValue;
+    long blocks = (ss.st[0].nk + G::KPW - 1) / G::KPW;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((rhs_stage_set_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, ss);
+    return hipGetLastError();
+}
+
+template <int N, int LR>
 static hipError_t launch_rhs_batched_NL(const StageArgs *args, const long *block0, int m, long blocks, hipStream_t s)
 {
     if (blocks <= 0 || m <= 0) return hipSuccess;
@@ -789,6 +816,23 @@ hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s)
     case 804: return launch_rhs_NL<8, 4>(a, s);
     case 1602: return launch_rhs_NL<16, 2>(a, s);
     case 1604: return launch_rhs_NL<16, 4>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs_stage_set(int N, int LR, const StageSet &ss, hipStream_t s)
+{
+    switch (N * 100 + LR) {
+    case 201: return launch_rhs_set_NL<2, 1>(ss, s);
+    case 202: return launch_rhs_set_NL<2, 2>(ss, s);
+    case 401: return launch_rhs_set_NL<4, 1>(ss, s);
+    case 402: return launch_rhs_set_NL<4, 2>(ss, s);
+    case 404: return launch_rhs_set_NL<4, 4>(ss, s);
+    case 801: return launch_rhs_set_NL<8, 1>(ss, s);
+    case 802: return launch_rhs_set_NL<8, 2>(ss, s);
+    case 804: return launch_rhs_set_NL<8, 4>(ss, s);
+    case 1602: return launch_rhs_set_NL<16, 2>(ss, s);
+    case 1604: return launch_rhs_set_NL<16, 4>(ss, s);
     }
     return hipErrorInvalidValue;
 }
