@@ -743,6 +743,9 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
         c->y16 = dalloc<double>(SET_MAX * c->y1_stride);
     }
     c->err_partial = dalloc<double>(std::max(1024L, c->nk));  // one per stage block (>= 1 k-point each)
+    // SBEWH_FUSE_BELOW overrides the stage-grid size under which an attempt's
+    // stages share one launch (0: never, as for large grids)
+    if (const char *env = std::getenv("SBEWH_FUSE_BELOW")) c->fuse_below_blocks = std::atol(env);
     if (!member_of_scan) {
         HIPCHK(hipHostMalloc((void **)&c->err_host, sizeof(unsigned long long),
                              hipHostMallocMapped | hipHostMallocCoherent));

@@ -161,3 +161,38 @@ def test_replay_square_short_pulse():
     assert np.abs(rg["j"] - ro["j"]).max() < 1e-10 * scale
     assert rel(sg, so) < 1e-10
     check_error_norms(rg, ro)
+
+
+def test_stage_per_launch_equals_attempt_per_launch(monkeypatch):
+    """Grids of at least 512 stage blocks launch an attempt's six stages one by
+    one, smaller grids in one launch; on the same problem both give the same
+    bits (own step control), and the one-by-one path replays the oracle's
+    step sequence to 1e-10 like the other."""
+    say("[gpu] square2d 15x15: one launch per stage vs one per attempt")
+    cfg = cases.small_square(eps_rel=1e-7)
+    o = Oracle(cfg)
+    mu = o.fermi_level(cfg.occupied_bands)
+    st0 = o.initial_state(mu)
+    so = st0.copy()
+    ro = o.integrate(so, cfg.output_count)
+    monkeypatch.setenv("SBEWH_STAGE_LR", "1")   # the same layout both ways
+    res = {}
+    for fuse in ("0", "512"):
+        monkeypatch.setenv("SBEWH_FUSE_BELOW", fuse)
+        g = SBEContext(cfg)
+        s = st0.copy()
+        r = g.integrate(s, cfg.output_count)
+        res[fuse] = (r, s)
+        if fuse == "0":
+            sr = st0.copy()
+            rr = g.integrate(sr, cfg.output_count, replay=ro["log"])
+            scale = np.abs(ro["j"]).max()
+            assert np.abs(rr["j"] - ro["j"]).max() < 1e-10 * scale
+            assert rel(sr, so) < 1e-10
+            check_error_norms(rr, ro)
+        g.close()
+    (r0, s0), (r1, s1) = res["0"], res["512"]
+    say(f"  steps {r0['steps']} / {r1['steps']}")
+    assert r0["steps"] == r1["steps"]
+    assert np.array_equal(r0["j"], r1["j"])
+    assert np.array_equal(s0, s1)
