@@ -52,3 +52,39 @@ def test_full_size_fermi_initial_rhs_observe(name):
     j_g, j_o = g.observe(x), o.observe_j(x)
     assert np.abs(j_g - j_o).max() < 1e-12 * max(np.abs(j_o).max(), 1e-30)
     g.close()
+
+
+def test_full_size_propagation_invariants():
+    """Whole-pulse propagation of the 150 x 150 workload on the benchmark's
+    path (one launch per stage): the dynamics conserves every Tr rho_k
+    (commutator and dephasing are traceless) and keeps rho Hermitian; the
+    special slots follow dA/dt = -E, so A at the last output time equals
+    -int E dt (composite Gauss-Legendre quadrature of the pulse) to the
+    integrator's relative tolerance 1e-8 of the peak |A|; the current stays
+    finite."""
+    cfg = model.config("square2d")
+    say(f"[gpu] full size square2d: whole-pulse propagation, {cfg.Nk}")
+    g = SBEContext(cfg)
+    st = g.initial_state(g.fermi_level(cfg.occupied_bands))
+    tr0 = np.trace(g.rho(st), axis1=1, axis2=2).copy()
+    r = g.integrate(st, 64)
+    say(f"  {r['steps']} steps, {r['rhs_calls']} RHS")
+    rho = g.rho(st)
+    tr = np.trace(rho, axis1=1, axis2=2)
+    assert np.abs(tr - tr0).max() < 1e-10 * np.abs(tr0).max()
+    assert np.abs(rho - np.conj(np.swapaxes(rho, 1, 2))).max() < 1e-10 * np.abs(rho).max()
+    assert np.all(np.isfinite(r["j"]))
+    t0, t_last = g.time_window[0], r["t"][-1]
+    xs, ws = np.polynomial.legendre.leggauss(8)
+    edges = np.linspace(t0, t_last, 2001)
+    integral = np.zeros(3)
+    peak = 0.0   # largest |A| along the pulse: the scale of the integrator's tolerance
+    for a, b in zip(edges[:-1], edges[1:]):
+        for x, w in zip(xs, ws):
+            integral += 0.5 * (b - a) * w * g.E(0.5 * (a + b) + 0.5 * (b - a) * x)
+        peak = max(peak, np.abs(integral).max())
+    A = st[:3].real
+    say(f"  A(t_end) {A}, -int E {-integral}, peak |A| {peak:.3e}")
+    assert peak > 0.1
+    assert np.abs(A + integral).max() < 1e-8 * peak
+    g.close()
