@@ -160,7 +160,7 @@ struct sbe_ctx {
     double *tw6 = nullptr, *hm6 = nullptr, *Y6 = nullptr, *y16 = nullptr;
     long tw_stride = 0, hm_stride = 0, Y_stride = 0, y1_stride = 0;
     long fuse_below_blocks = 512;  // stage grids smaller than this run an attempt's stages in one launch
-    long set_min_blocks = 512;  // block target of a set's contraction launch (measured best at c2)
+    long set_min_blocks = 512;  // block target of a set's contraction launch (best on the 150 x 150 2-D model)
     bool ucache_valid = false;
     double *err_partial = nullptr;
     unsigned long long *err_host = nullptr;      // pinned, mapped, fine-grained
@@ -325,11 +325,11 @@ struct sbe_ctx {
         return a;
     }
 
-    // The phase sums of the m stage inputs of one attempt (times of E[q]), all
+    // The phase sums of the m stage inputs of one attempt (fields E[q]), all
     // formed before the first stage runs: they depend on rho only through the
-    // special slots A, and the A slot of a derivative computed in this attempt
-    // is -E at its stage time, known here (made[j] = stage whose output k_j is,
-    // or -1 for a derivative already in memory).
+    // special slots A, and the A slot of a derivative that an earlier stage of
+    // this attempt computes (outs[r]) is -E[r], known here; derivatives that
+    // are no stage's output (k1) are read from memory.
     void phase_set(const StageInput *ins, const double (*E)[3], const double *const *outs, int m, hipStream_t s)
     {
         // the prep launch goes out before the contraction is planned: the GPU
