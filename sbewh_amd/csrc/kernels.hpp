@@ -226,19 +226,10 @@ hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s);
 // host memory) with a system-scope store.
 hipError_t launch_error_final(const double *partial, long nb, unsigned long long *out, hipStream_t s);
 
-// per-instance error norms: entry j covers state S_j of its instance and
-// writes its norm to *err_out (device); partial has >= sum_j blocks_j doubles
-struct ErrEntry {
-    const double *x;
-    const double *k[6];
-    double e[6];
-    double ea, er, dt;
-    long S;
-    double *err_out;
-};
-constexpr int ERR_CHUNK = 2048;   // state entries per block of the batched norm
-hipError_t launch_error_norm_batched(const ErrEntry *d_entries, const long *d_block0, int m, long blocks,
-                                     double *partial, hipStream_t s);
+// the same for the instances of a batched last stage: out[j] = max of
+// partial[block0[j] .. block0[j+1]) (device arrays; the last range ends at blocks)
+hipError_t launch_error_final_batched(const long *d_block0, int m, long blocks, const double *partial, double *out,
+                                     hipStream_t s);
 
 // ---- observer (observer_kernels.hip) -------------------------------------
 struct ObsArgs {

@@ -1,7 +1,7 @@
 // Dormand-Prince error norm (FP64).  The stage inputs x + (dt b_1) k_1 + ...
-// are formed by the stage kernel that consumes them, and a single instance's
-// error terms by its last stage kernel (per-block maxima, reduced here); the
-// batched form below covers the instances of a scan.
+// are formed by the stage kernel that consumes them, and the error terms by
+// the attempt's last stage kernel (per-block maxima, reduced here: one
+// instance, or the instances of a batched launch).
 //
 // Boost.Odeint's runge_kutta_dopri5 forms the error estimate as
 // sum_j (dt dc_j) k_j, normalised by
@@ -13,16 +13,6 @@
 #include "kernels.hpp"
 
 namespace sbe {
-
-// relative error of state entry i
-template <class A>
-__device__ __forceinline__ double err_term(const A &a, long i)
-{
-    double2 kv[6];
-#pragma unroll
-    for (int j = 0; j < 6; j++) kv[j] = reinterpret_cast<const double2 *>(a.k[j])[i];
-    return err_entry(reinterpret_cast<const double2 *>(a.x)[i], kv, a.e, a.ea, a.er, a.dt);
-}
 
 // max over the 256 threads of a block (result valid in thread 0)
 __device__ __forceinline__ double block_max(double v)
@@ -48,31 +38,18 @@ __global__ __launch_bounds__(256) void error_final_kernel(const double *partial,
                            __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// batched pass 1: block lb of entry j covers entries [lb, lb + 1) * ERR_CHUNK of its state
-__global__ __launch_bounds__(256) void error_batched_kernel(const ErrEntry *e, const long *block0, int m,
-                                                            double *partial)
-{
-    const long gb = blockIdx.x;
-    const int j = find_entry(block0, m, gb);
-    const ErrEntry q = e[j];
-    const long i0 = (gb - block0[j]) * ERR_CHUNK;
-    const long i1 = i0 + ERR_CHUNK < q.S ? i0 + ERR_CHUNK : q.S;
-    double v = 0;
-    for (long i = i0 + threadIdx.x; i < i1; i += 256) v = fmax(v, err_term(q, i));
-    v = block_max(v);
-    if (threadIdx.x == 0) partial[gb] = v;
-}
-
-// batched pass 2: block j reduces the partials of entry j into *err_out
-__global__ __launch_bounds__(256) void error_batched_final_kernel(const ErrEntry *e, const long *block0, int m,
-                                                                  long blocks, const double *partial)
+// per-instance final reduction of a batched last stage's per-block maxima:
+// block j reduces partial[block0[j] .. block0[j+1]) (the last entry up to
+// blocks) into out[j]
+__global__ __launch_bounds__(256) void error_final_batched_kernel(const long *block0, int m, long blocks,
+                                                                  const double *partial, double *out)
 {
     const int j = blockIdx.x;
     const long b1 = j + 1 < m ? block0[j + 1] : blocks;
     double v = 0;
     for (long b = block0[j] + threadIdx.x; b < b1; b += 256) v = fmax(v, partial[b]);
     v = block_max(v);
-    if (threadIdx.x == 0) *e[j].err_out = v;
+    if (threadIdx.x == 0) out[j] = v;
 }
 
 hipError_t launch_error_final(const double *partial, long nb, unsigned long long *out, hipStream_t s)
@@ -82,15 +59,11 @@ hipError_t launch_error_final(const double *partial, long nb, unsigned long long
     return hipGetLastError();
 }
 
-hipError_t launch_error_norm_batched(const ErrEntry *d_entries, const long *d_block0, int m, long blocks,
-                                     double *partial, hipStream_t s)
+hipError_t launch_error_final_batched(const long *d_block0, int m, long blocks, const double *partial, double *out,
+                                     hipStream_t s)
 {
     if (m <= 0 || blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(error_batched_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_entries, d_block0, m, partial);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(error_batched_final_kernel, dim3((unsigned)m), dim3(256), 0, s, d_entries, d_block0, m, blocks,
-                       (const double *)partial);
+    hipLaunchKernelGGL(error_final_batched_kernel, dim3((unsigned)m), dim3(256), 0, s, d_block0, m, blocks, partial, out);
     return hipGetLastError();
 }
 

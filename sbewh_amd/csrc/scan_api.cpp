@@ -39,6 +39,10 @@ struct RhsReq {
     double *dxdt;
     double t;
     double *xout;
+    // last stage of an attempt: the error terms' coefficients of k1, k3..k7
+    bool err = false;
+    double err_e[6] = {};
+    double err_dt = 0;
 };
 
 }  // namespace
@@ -57,8 +61,10 @@ struct sbe_scan {
     // table arena: pinned host staging mirrored by device memory
     char *h_arena = nullptr, *d_arena = nullptr;
     size_t arena_cap = 0, arena_used = 0, arena_flushed = 0;
-    double *d_partial = nullptr;  // batched error partials
+    double *d_partial = nullptr;  // error-norm maxima per stage block of the last stage
     long partial_cap = 0;
+    long *stage_b0 = nullptr;     // block offsets of the last batched stage launch (device, arena)
+    long stage_blocks = 0;
     double *d_err = nullptr, *h_err = nullptr;
     double *d_jout = nullptr;    // [instance][output][3]
     long jout_cap = 0;
@@ -200,9 +206,23 @@ struct sbe_scan {
             b0[j] = tot;
             tot += (args[j].nk + kpb - 1) / kpb;
         }
+        for (int j = 0; j < m; j++)
+            if (rq[j].err) {
+                // the attempt's error terms, per stage block, in this (last) stage
+                if (tot > partial_cap) fail(SBE_ERR_ARG, "error partial buffer too small");
+                const sbe_ctx &c = *rq[j].c;
+                StageArgs &a = args[j];
+                a.err_partial = d_partial + b0[j];
+                for (int q = 0; q < 6; q++) a.err_e[q] = rq[j].err_e[q];
+                a.err_ea = c.prm.eps_abs;
+                a.err_er = c.prm.eps_rel;
+                a.err_dt = rq[j].err_dt;
+            }
         StageArgs *da = add(args.data(), m);
         long *db = add(b0.data(), m);
         flush();
+        stage_b0 = db;
+        stage_blocks = tot;
         HIPCHK(launch_rhs_stage_batched(N, lr, da, db, m, tot, stream));
         for (int j = 0; j < m; j++) {
             if (args[j].dephase) rq[j].c->ucache_valid = true;
@@ -329,38 +349,17 @@ struct sbe_scan {
                     r.in.nt = 5;
                     r.dxdt = c.st[7];
                     r.xout = c.st[9];
+                    const double dc[6] = {DC1, DC3, DC4, DC5, DC6, DC7};
+                    r.err = true;
+                    for (int q = 0; q < 6; q++) r.err_e[q] = dt * dc[q];
+                    r.err_dt = dt;
                 }
                 r.t = (s < 4) ? t + dt * ct[s] : t + dt;
             }
             rhs(rq);
         }
-        std::vector<ErrEntry> ee(m);
-        std::vector<long> b0(m);
-        long tot = 0;
-        for (int j = 0; j < m; j++) {
-            sbe_ctx &c = *inst[ids[j]];
-            const double dt = ctl[ids[j]].dt;
-            ErrEntry &e = ee[j];
-            e.x = c.st[0];
-            const int kk[6] = {1, 3, 4, 5, 6, 7};
-            const double dc[6] = {DC1, DC3, DC4, DC5, DC6, DC7};
-            for (int q = 0; q < 6; q++) {
-                e.k[q] = c.st[kk[q]];
-                e.e[q] = dt * dc[q];
-            }
-            e.ea = c.prm.eps_abs;
-            e.er = c.prm.eps_rel;
-            e.dt = dt;
-            e.S = c.S;
-            e.err_out = d_err + j;
-            b0[j] = tot;
-            tot += (c.S + ERR_CHUNK - 1) / ERR_CHUNK;
-        }
-        if (tot > partial_cap) fail(SBE_ERR_ARG, "error partial buffer too small");
-        ErrEntry *de = add(ee.data(), m);
-        long *db = add(b0.data(), m);
-        flush();
-        HIPCHK(launch_error_norm_batched(de, db, m, tot, d_partial, stream));
+        // the last stage left each instance's error maxima per stage block
+        HIPCHK(launch_error_final_batched(stage_b0, m, stage_blocks, d_partial, d_err, stream));
         HIPCHK(hipMemcpyAsync(h_err, d_err, sizeof(double) * m, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
@@ -395,7 +394,7 @@ int scan_guarded(sbe_scan *s, F &&f)
 size_t arena_bytes_per_instance()
 {
     return sizeof(TwiddleEntry) + 2 * sizeof(ContractStage) + 3 * sizeof(long) + sizeof(StageArgs) +
-           sizeof(ObsArgs) + sizeof(ErrEntry) + 6 * 256;
+           sizeof(ObsArgs) + 6 * 256;
 }
 
 }  // namespace
@@ -435,7 +434,7 @@ int sbe_scan_create(const sbe_model *model, const sbe_grid *grid, const sbe_para
             c->stream = sc->stream;
             if (c->geom.W[c->dim - 1] > 16 || (c->dim > 1 && c->geom.W[0] > 16) || (c->dim > 2 && c->geom.W[1] > 16))
                 fail(SBE_ERR_ARG, "batched scans support R extents up to 16 per axis");
-            partial_blocks += (c->S + ERR_CHUNK - 1) / ERR_CHUNK;
+            partial_blocks += c->nk;  // stage blocks hold at least one k-point
         }
         sc->n = sc->inst[0]->n;
         sc->N = sc->inst[0]->N;
