@@ -268,8 +268,8 @@ def main():
         print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md but not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding the k-points over "
+              "several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     if args.scan:
         scan_bench(args)
