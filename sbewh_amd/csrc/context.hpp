@@ -163,6 +163,7 @@ struct sbe_ctx {
     long set_min_blocks = 512;  // block target of a set's contraction launch (best on the 150 x 150 2-D model)
     bool ucache_valid = false;
     double *err_partial = nullptr;
+    unsigned *err_ticket = nullptr;              // ticket of the fused attempt's last block (zero between launches)
     unsigned long long *err_host = nullptr;      // pinned, mapped, fine-grained
     unsigned long long *err_host_dev = nullptr;  // its device address
     double *partial = nullptr;
@@ -430,14 +431,21 @@ struct sbe_ctx {
         for (int q = 0; q < 6; q++) E_at(ts[q], E[q]);
         // error coefficients of k1, k3, k4, k5, k6, k7
         const double err_e[6] = {dt * DC1, dt * DC3, dt * DC4, dt * DC5, dt * DC6, dt * DC7};
+        const long blocks = lr ? (nk + stage_k_per_block(N, lr) - 1) / stage_k_per_block(N, lr) : 0;
+        const bool fused = lr && !timing && blocks < fuse_below_blocks;
+        // the last kernel of the attempt overwrites this with the norm
+        *reinterpret_cast<volatile unsigned long long *>(err_host) = ERR_PENDING;
         // the six phase sums first, then the six stage kernels
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
         phase_set(ins, E, outs, 6, s);
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
-        if (lr && !timing && (nk + stage_k_per_block(N, lr) - 1) / stage_k_per_block(N, lr) < fuse_below_blocks) {
-            // a grid this small is launch-bound: the six stages in one launch
+        if (fused) {
+            // a grid this small is launch-bound: the six stages in one launch,
+            // whose last block also reduces the error norm
             StageSet ss{};
             ss.m = 6;
+            ss.ticket = err_ticket;
+            ss.err_out = err_host_dev;
             for (int q = 0; q < 6; q++) {
                 ss.st[q] = stage_args(ins[q], outs[q], E[q], last_set(q), q == 5 ? xn : nullptr,
                                       q == 5 ? err_e : nullptr, dt);
@@ -457,9 +465,11 @@ struct sbe_ctx {
             }
         }
         // the last stage kernel left one error-norm partial per block
-        *reinterpret_cast<volatile unsigned long long *>(err_host) = ERR_PENDING;
-        HIPCHK(launch_error_final(err_partial, (nk + stage_k_per_block(N, lr) - 1) / stage_k_per_block(N, lr),
-                                  err_host_dev, s));
+        // (the stage layout, and with it the block count, is known only after
+        // the first stage launch when the tuner has not run yet)
+        if (!fused)
+            HIPCHK(launch_error_final(err_partial, (nk + stage_k_per_block(N, lr) - 1) / stage_k_per_block(N, lr),
+                                      err_host_dev, s));
         return wait_error(s);
     }
 
@@ -505,6 +515,7 @@ struct sbe_ctx {
         if (kmap) (void)hipFree(kmap);
         if (kphase) (void)hipFree(kphase);
         if (err_partial) (void)hipFree(err_partial);
+        if (err_ticket) (void)hipFree(err_ticket);
         if (err_host) (void)hipHostFree(err_host);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -743,6 +754,8 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
         c->y16 = dalloc<double>(SET_MAX * c->y1_stride);
     }
     c->err_partial = dalloc<double>(std::max(1024L, c->nk));  // one per stage block (>= 1 k-point each)
+    c->err_ticket = dalloc<unsigned>(1);
+    HIPCHK(hipMemset(c->err_ticket, 0, sizeof(unsigned)));
     // SBEWH_FUSE_BELOW overrides the stage-grid size under which an attempt's
     // stages share one launch (0: never, as for large grids)
     if (const char *env = std::getenv("SBEWH_FUSE_BELOW")) c->fuse_below_blocks = std::atol(env);

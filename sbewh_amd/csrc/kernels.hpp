@@ -193,6 +193,11 @@ hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s);
 struct StageSet {
     StageArgs st[SET_MAX];
     int m;
+    // optional: the last stage's per-block error maxima are reduced by the
+    // block that finishes last (a ticket counter, zero between launches) and
+    // the norm is stored into err_out (mapped host memory)
+    unsigned *ticket;
+    unsigned long long *err_out;
 };
 hipError_t launch_rhs_stage_set(int N, int LR, const StageSet &ss, hipStream_t s);
 // LR values compiled for N (returns the count written to lr_out)

@@ -1,7 +1,8 @@
 // Dormand-Prince error norm (FP64).  The stage inputs x + (dt b_1) k_1 + ...
 // are formed by the stage kernel that consumes them, and the error terms by
 // the attempt's last stage kernel (per-block maxima, reduced here: one
-// instance, or the instances of a batched launch).
+// instance, or the instances of a batched launch; an attempt whose six stages
+// share one launch reduces them in that launch's last block).
 //
 // Boost.Odeint's runge_kutta_dopri5 forms the error estimate as
 // sum_j (dt dc_j) k_j, normalised by

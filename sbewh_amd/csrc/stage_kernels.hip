@@ -756,6 +756,28 @@ __global__ __launch_bounds__(64) void rhs_stage_set_kernel(
         __threadfence_block();
         wsync();
     }
+    if (ss.ticket) {
+        // thread 0 stored this block's error maximum; its ticket releases it.
+        // The block that takes the last ticket reduces all of them, resets the
+        // counter for the next launch and stores the norm's bit pattern with a
+        // system-scope store (no block waits for another)
+        unsigned t = 0;
+        if (threadIdx.x == 0) t = __hip_atomic_fetch_add(ss.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        t = __shfl(t, 0);
+        if (t + 1 == gridDim.x) {
+            __threadfence();
+            const double *part = ss.st[ss.m - 1].err_partial;
+            double v = 0;
+            for (unsigned i = threadIdx.x; i < gridDim.x; i += 64)
+                v = fmax(v, __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            v = seg_max<64>(v);
+            if (threadIdx.x == 0) {
+                __hip_atomic_store(ss.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(ss.err_out, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+    }
 }
 
 template <int N, int LR>
@@ -766,6 +788,7 @@ static hipError_t launch_rhs_set_NL(const StageSet &ss, hipStream_t s)
     if (ss.m > SET_MAX) return hipErrorInvalidValue;
     for (int q = 1; q < ss.m; q++)
         if (ss.st[q].nk != ss.st[0].nk) return hipErrorInvalidValue;
+    if (ss.ticket && (!ss.err_out || !ss.st[ss.m - 1].err_partial)) return hipErrorInvalidValue;
     long blocks = (ss.st[0].nk + G::KPW - 1) / G::KPW;
     if (blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL((rhs_stage_set_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, ss);
