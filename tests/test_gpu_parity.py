@@ -196,3 +196,37 @@ def test_stage_per_launch_equals_attempt_per_launch(monkeypatch):
     assert r0["steps"] == r1["steps"]
     assert np.array_equal(r0["j"], r1["j"])
     assert np.array_equal(s0, s1)
+
+
+def test_one_block_attempt_kernel(monkeypatch):
+    """A grid of a single stage block: that block is the one that takes the
+    last ticket of the fused attempt kernel and reduces the error norm alone.
+    Same bits as one launch per stage (own step control), and the oracle's
+    step sequence replayed to 1e-10 with matching error norms."""
+    say("[gpu] chain1d 16 k (one stage block): one launch per attempt vs one per stage")
+    cfg = cases.chain(Nk=(16, 1, 1))
+    o = Oracle(cfg)
+    mu = o.fermi_level(cfg.occupied_bands)
+    st0 = o.initial_state(mu)
+    so = st0.copy()
+    ro = o.integrate(so, cfg.output_count)
+    res = {}
+    for fuse in ("512", "0"):
+        monkeypatch.setenv("SBEWH_FUSE_BELOW", fuse)
+        g = SBEContext(cfg)
+        s = st0.copy()
+        r = g.integrate(s, cfg.output_count)
+        res[fuse] = (r, s)
+        if fuse == "512":
+            sr = st0.copy()
+            rr = g.integrate(sr, cfg.output_count, replay=ro["log"])
+            scale = np.abs(ro["j"]).max()
+            assert np.abs(rr["j"] - ro["j"]).max() < 1e-10 * scale
+            assert rel(sr, so) < 1e-10
+            check_error_norms(rr, ro)
+        g.close()
+    (r0, s0), (r1, s1) = res["512"], res["0"]
+    say(f"  steps {r0['steps']} / {r1['steps']} (oracle {ro['steps']})")
+    assert r0["steps"] == r1["steps"]
+    assert np.array_equal(r0["j"], r1["j"])
+    assert np.array_equal(s0, s1)
