@@ -44,6 +44,12 @@ def _compile(src: str, force: bool) -> str:
     # the host controller's time arithmetic is kept free of FMA contraction so
     # that output times are the plain IEEE sums t0 + step * dt_out
     extra = ["-ffp-contract=off"] if src.endswith(".cpp") else []
+    # the stage kernels are issue-bound with a third of their cycles waiting;
+    # the ILP-oriented machine scheduler hides more of that latency at the same
+    # occupancy (150 x 150 2-D model: 116.8 -> 113.7 us per stage launch;
+    # 48^3 3-D model: 2.99 -> 2.95 ms)
+    if src == "stage_kernels.hip":
+        extra += ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
     cmd = [HIPCC] + COMMON + extra + lang + ["-c", os.path.join(SRC, src), "-o", obj + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
