@@ -37,7 +37,7 @@ def _newer(a: str, deps: list) -> bool:
 def _compile(src: str, force: bool) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     deps = [os.path.join(SRC, src)] + [os.path.join(SRC, h) for h in HEADERS] + \
-        [os.path.join(INCLUDE, "sbewh_hip.h"), os.path.join(INCLUDE, "sbewh_types.h")]
+        [os.path.join(INCLUDE, "sbewh_hip.h"), os.path.join(INCLUDE, "sbewh_types.h"), os.path.abspath(__file__)]
     if not force and _newer(obj, deps):
         return obj
     lang = ["-x", "hip"]
