@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark of the SBE RHS + Dormand-Prince propagation on one MI355X.
 
-Workload (square2d): 2-D 8-band seeded tight-binding model on a square lattice
-(a = 3 A, |R_i| <= 6: 169 cells), 150 x 150 k-points (22 500), elliptic sin^2
-pulse (3 um, ellipticity 0.3, 10 MV/cm), T2 = 10 fs, soothing width 25 meV,
-5 Jacobi sweeps, FP64.
+Headline workload (cubic3d, the largest single-GPU configuration): seeded 3-D
+16-band tight-binding model on a cubic lattice (a = 4 A, R in [-4,4]^3: 729
+cells), 48^3 = 110 592 k-points, linear sin^2 pulse (2 um, 10 cycles,
+8 MV/cm), T2 = 8 fs, soothing width 25 meV, <= 5 Jacobi sweeps, FP64.  The
+2-D 8-band 150 x 150 workload (square2d) and the 1-D two-band chain
+(chain1d) are measured the same way and reported as extra keys of the line.
 
 One step = one Dormand-Prince 5(4) step attempt over the whole device-resident
 state: 6 RHS evaluations (partial phase sum + the fused stage kernel, which
@@ -12,9 +14,9 @@ forms the stage input, finishes the phase sum and applies commutator, Jacobi
 and dephasing) and the error-norm reduction read back by the host
 controller.  value = N_k * 6 * K / seconds (k-point RHS evaluations per
 second).  The CPU baseline is the C/OpenMP oracle running the same attempts on
-the host cores.
+the host cores (-O3, and -O1 as the reference builds).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cubic3d|square2d|chain1d]
     python bench.py --scan [--plain-sample P]   (spectra/hour, batched vs plain)
     python bench.py --surrogate                 (spectra/hour, surrogate + fallback vs plain)
 """
@@ -71,16 +73,41 @@ def sync():
         pass
 
 
+def physical_cores():
+    """physical cores of the host (unique (core, socket) pairs of lscpu -p)"""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        pairs = {ln for ln in out.splitlines() if ln and not ln.startswith("#")}
+        return len(pairs) or None
+    except (OSError, subprocess.CalledProcessError):
+        return None
+
+
 def cpu_baseline(cfg, state, t, dt, budget_s: float):
-    from oracle.oracle_py import Oracle
+    """The C/OpenMP oracle (a restatement of PropagatorCPU + the dopri5 driver)
+    timing the same attempts on the host cores, at -O3 and at the reference's
+    own -O1; threads pinned close (OMP_PROC_BIND is read when the OpenMP
+    runtime starts, i.e. at the oracle's first load in this process)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from oracle.oracle_py import LIB_PATH_O1, Oracle
     o = Oracle(cfg)
     one = o.bench_attempts(state, t, dt, 1)
     count = max(2, int(budget_s / max(one, 1e-6)))
     el = o.bench_attempts(state, t, dt, count)
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    return {"value": o.num_k * 6 * count / el, "unit": "k-RHS/s", "cores": cores, "kind": "port",
-            "sample": f"{count} dopri5 attempts (6 RHS each) on the full {o.num_k}-k state, C/OpenMP oracle, "
-                      f"{el:.1f} s"}
+    threads = o.num_threads
+    o.close()
+    o1 = Oracle(cfg, lib_path=LIB_PATH_O1)
+    el1 = o1.bench_attempts(state, t, dt, 1)
+    o1.close()
+    nk = o.num_k
+    return {"value": nk * 6 * count / el, "unit": "k-RHS/s", "cores": threads, "kind": "port",
+            "sample": f"{count} dopri5 attempts (6 RHS each) on the full {nk}-k state, C/OpenMP oracle -O3, "
+                      f"{threads} threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), {el:.1f} s",
+            "value_O1": nk * 6 / el1, "sample_O1": f"1 attempt at -O1 (the reference's CMAKE_CXX_FLAGS), "
+                                                   f"{el1:.1f} s",
+            "host_physical_cores": physical_cores()}
 
 
 def spectra_of(t, j, omega0):
@@ -244,8 +271,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="square2d", choices=("square2d", "chain1d", "cubic3d"),
-                    help="workload of the k-RHS/s measurement (default: the 2-D 8-band benchmark)")
+    ap.add_argument("--config", default="cubic3d", choices=("square2d", "chain1d", "cubic3d"),
+                    help="headline workload of the k-RHS/s measurement (default: the largest, the 3-D "
+                         "16-band model on 48^3 k); the other two are measured too, as extra keys")
+    ap.add_argument("--headline-only", action="store_true", help="measure only the --config workload")
     ap.add_argument("--scan", action="store_true",
                     help="spectra/hour of the 64-pulse batched scan versus the plain solver")
     ap.add_argument("--scan-count", type=int, default=64)
@@ -278,71 +307,91 @@ def main():
         surrogate_bench(args)
         return
 
+    head = measure(args.config, args.steps, args.warmup, cpu_seconds=None if args.no_cpu else args.cpu_seconds)
+    out = head["line"]
+    # the other two workloads, measured the same way (no CPU leg), as extra keys
+    if not args.headline_only:
+        for other in ("cubic3d", "square2d", "chain1d"):
+            if other != args.config:
+                r = measure(other, max(args.steps, 50), args.warmup, cpu_seconds=None)
+                out[other] = {k: r["line"][k] for k in ("value", "unit", "ms_per_step", "roofline", "config")}
+    print(json.dumps(out), flush=True)
+
+
+WORKLOADS = {
+    "square2d": "square2d: seeded 2-D 8-band model, 169 R, 150x150 k, elliptic sin2 3 um eps 0.3 10 MV/cm, "
+                "T2 10 fs, rtol 1e-8, Jacobi <= 5 sweeps (warm-started)",
+    "chain1d": "chain1d: seeded 1-D 2-band chain, 9 R, 256 k, sin2 3.2 um 8 cycles 5 MV/cm, T2 5 fs, rtol 1e-8",
+    "cubic3d": "cubic3d: seeded 3-D 16-band model, 729 R, 48^3 k, sin2 2 um 10 cycles 8 MV/cm, T2 8 fs, "
+               "rtol 1e-8, Jacobi <= 5 sweeps (warm-started)",
+}
+
+
+def measure(name, steps, warmup, cpu_seconds):
+    """k-RHS/s of one workload: `steps` adaptive dopri5 attempts after `warmup`,
+    the stage kernel's launch time from HIP events, and its roofline"""
     from sbewh_amd import model
     from sbewh_amd.propagator import SBEContext
 
-    cfg = model.config(args.config)
+    cfg = model.config(name)
     ctx = SBEContext(cfg)
     mu = ctx.fermi_level(cfg.occupied_bands)
     state = ctx.initial_state(mu)
     t0, t1 = ctx.time_window
     t, dt = 0.5 * t0, 0.5   # rising edge of the pulse: E(t) != 0
     n, S, nk = cfg.model.num_wann, cfg.sweeps, ctx.num_k
+    print(f"[bench] {name}: {nk} k, n = {n}, {steps} attempts", file=sys.stderr, flush=True)
 
     sync()
-    r = ctx.bench_steps(state, t, dt, args.warmup, args.steps, kernel_timing=True)
+    r = ctx.bench_steps(state, t, dt, warmup, steps, kernel_timing=True)
     sync()
     secs = r["ms"] / 1e3
     value = nk * r["rhs"] / secs
     stage_avg_s = r["stage_ms"] / 1e3
     phase_avg_s = r["phase_ms"] / 1e3
     flops_launch = stage_flops_per_k(n, S, ctx.last_axis_cells) * nk
+    flops_launch_table = stage_flops_per_k(n, S, 0) * nk
     bytes_launch = stage_bytes_per_k(n) * nk
     Npad = 2 if n <= 2 else 4 if n <= 4 else 8 if n <= 8 else 16
-    # the lower roof of the stage kernel's algorithmic figures bounds it
+    # the lower roof of the stage kernel's algorithmic figures bounds it; the
+    # kernel runs on the FP64 pipes (78.6 TFLOP/s dense, vector and matrix)
     if flops_launch / (PEAK_FP64_TFLOPS * 1e12) >= bytes_launch / (PEAK_HBM_GBS * 1e9):
-        roof = {"bound": "mfma", "achieved": flops_launch / stage_avg_s / 1e12, "peak": PEAK_FP64_TFLOPS,
+        roof = {"bound": "fp64", "achieved": flops_launch_table / stage_avg_s / 1e12, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s"}
     else:
         roof = {"bound": "hbm", "achieved": bytes_launch / stage_avg_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"stage_kernel_{args.config}_pmc.md"))
-    workloads = {
-        "square2d": "square2d: seeded 2-D 8-band model, 169 R, 150x150 k, elliptic sin2 3 um eps 0.3 10 MV/cm, "
-                    "T2 10 fs, rtol 1e-8, Jacobi <= 5 sweeps (warm-started)",
-        "chain1d": "chain1d: seeded 1-D 2-band chain, 9 R, 256 k, sin2 3.2 um 8 cycles 5 MV/cm, T2 5 fs, rtol 1e-8",
-        "cubic3d": "cubic3d: seeded 3-D 16-band model, 729 R, 48^3 k, sin2 2 um 10 cycles 8 MV/cm, T2 8 fs, "
-                   "rtol 1e-8, Jacobi <= 5 sweeps (warm-started)",
-    }
-
-    out = {
+    if roof["bound"] == "fp64":
+        # the same launch time credited with the fused last-axis contraction too
+        roof["frac_with_contraction"] = flops_launch / stage_avg_s / 1e12 / PEAK_FP64_TFLOPS
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"stage_kernel_{name}_pmc.md"))
+    line = {
         "metric": METRIC,
         "value": value,
         "unit": "k-RHS/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["ms"] / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": r["ms"] / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workloads[args.config] + "; step = one adaptive dopri5 attempt (6 RHS) "
-                                                       "advancing the state",
+        "config": {"workload": WORKLOADS[name] + "; step = one adaptive dopri5 attempt (6 RHS) "
+                                                "advancing the state",
                    "num_k": nk, "bands": n, "cells": cfg.model.num_cells, "sweeps": S,
                    "rhs_evals_timed": r["rhs"]},
         "roofline": dict(roof, traffic=traffic, kernel=f"rhs_stage_kernel<{Npad}>", launch_ms=stage_avg_s * 1e3,
-                         algorithmic_flop_per_launch=flops_launch, algorithmic_bytes_per_launch=bytes_launch),
+                         algorithmic_flop_per_launch=flops_launch_table,
+                         algorithmic_bytes_per_launch=bytes_launch),
         "breakdown_ms_per_rhs": {"stage": stage_avg_s * 1e3, "phase_sum": phase_avg_s * 1e3,
                                  "all": r["ms"] / r["rhs"]},
         "stage_tuning": ctx.stage_tuning(),
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, state, t, dt, args.cpu_seconds)
-    else:
-        out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    line["cpu_baseline"] = cpu_baseline(cfg, state, t, dt, cpu_seconds) if cpu_seconds else None
+    ctx.close()
+    return {"line": line}
 
 
 if __name__ == "__main__":

@@ -63,6 +63,13 @@ int sbe_time_window(const sbe_ctx *ctx, double *t0, double *t1);
 /* E(t) of the pulse (Pulse3D::getE, code/src/pulse/Pulse3D.cpp:15-21) */
 int sbe_pulse_E(const sbe_ctx *ctx, double t, double E[3]);
 
+/* E(t) and the window [t0, t1] of a set of sub-pulses without a context
+ * (host only, no GPU needed): Pulse3D::getE / getStartTime / getEndTime,
+ * code/src/pulse/Pulse3D.cpp:15-21,42-62, over the closed forms of
+ * code/src/pulse/{Sin2Pulse1D,GaussianPulse1D,PureGauss1D,Sin2RampPulse1D}.cpp. */
+int sbe_pulse_field(const sbe_subpulse *pulses, int num_pulses, double t, double E[3]);
+int sbe_pulse_window(const sbe_subpulse *pulses, int num_pulses, double *t0, double *t1);
+
 /* Fermi level from all eigenvalues at A = 0 (getFermiLevelCPU / calcFermiLevelBase,
  * code/src/main/runSBE.cpp:97-155).  below_gap != 0 selects occupiedBelowBandGap. */
 int sbe_fermi_level(sbe_ctx *ctx, int occupied_bands, int below_gap, double *mu);

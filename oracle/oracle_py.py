@@ -16,6 +16,8 @@ from sbewh_amd.ctypes_types import Inputs, SbeModel, SbeGrid, SbeParams, SbeSubp
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "libsbe_oracle.so")
+# the same source built at -O1, the reference's own optimisation level
+LIB_PATH_O1 = os.path.join(HERE, "build", "libsbe_oracle_O1.so")
 
 
 def build(force: bool = False) -> str:
@@ -25,15 +27,14 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(path: str = LIB_PATH):
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         P = C.POINTER
         vp = C.c_void_p
         L.oracle_create.restype = vp
@@ -68,8 +69,10 @@ def lib():
                                        P(SbeSteplog), P(SbeSteplog), P(C.c_long)]
         L.oracle_bench_attempts.restype = C.c_double
         L.oracle_bench_attempts.argtypes = [vp, P(C.c_double), C.c_double, C.c_double, C.c_int]
-        _lib = L
-    return _lib
+        L.oracle_num_threads.restype = C.c_int
+        L.oracle_num_threads.argtypes = []
+        _libs[path] = L
+    return _libs[path]
 
 
 def _v3(x):
@@ -88,9 +91,9 @@ def jacobi(A: np.ndarray, sweeps: int):
 class Oracle:
     """CPU restatement of PropagatorCPU + ObserverCPU + the dopri5 driver."""
 
-    def __init__(self, cfg, phase_method: str = "separable"):
+    def __init__(self, cfg, phase_method: str = "separable", lib_path: str = LIB_PATH):
         self.inp = Inputs(cfg)
-        L = lib()
+        self.L = L = lib(lib_path)
         self.h = L.oracle_create(C.byref(self.inp.model), C.byref(self.inp.grid), C.byref(self.inp.params),
                                  self.inp.pulses, self.inp.num_pulses)
         if not self.h:
@@ -103,7 +106,7 @@ class Oracle:
 
     def close(self):
         if self.h:
-            lib().oracle_destroy(self.h)
+            self.L.oracle_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -115,46 +118,46 @@ class Oracle:
     def set_pulses(self, pulses):
         from sbewh_amd.ctypes_types import pulse_array
         arr = pulse_array(pulses)
-        if lib().oracle_set_pulses(self.h, arr, len(pulses)):
-            raise RuntimeError(lib().oracle_last_error().decode())
+        if self.L.oracle_set_pulses(self.h, arr, len(pulses)):
+            raise RuntimeError(self.L.oracle_last_error().decode())
 
     def set_phase_method(self, method: str):
-        lib().oracle_set_phase_method(self.h, 0 if method == "direct" else 1)
+        self.L.oracle_set_phase_method(self.h, 0 if method == "direct" else 1)
 
     @property
     def nf(self):
         a = (C.c_int * 3)()
-        lib().oracle_get_nf(self.h, a)
+        self.L.oracle_get_nf(self.h, a)
         return tuple(a)
 
     def kfrac(self, k: int) -> np.ndarray:
         a = np.zeros(3)
-        lib().oracle_k_frac(self.h, k, dptr(a))
+        self.L.oracle_k_frac(self.h, k, dptr(a))
         return a
 
     def E(self, t):
         a = np.zeros(3)
-        lib().oracle_pulse_E(self.h, t, dptr(a))
+        self.L.oracle_pulse_E(self.h, t, dptr(a))
         return a
 
     def A(self, t):
         a = np.zeros(3)
-        lib().oracle_pulse_A(self.h, t, dptr(a))
+        self.L.oracle_pulse_A(self.h, t, dptr(a))
         return a
 
     @property
     def t_start(self):
-        return lib().oracle_start_time(self.h)
+        return self.L.oracle_start_time(self.h)
 
     @property
     def t_end(self):
-        return lib().oracle_end_time(self.h)
+        return self.L.oracle_end_time(self.h)
 
     def hk_point(self, kfrac, E=(0, 0, 0)):
         n = self.n
         H0 = np.zeros((n, n), complex)
         HE = np.zeros((n, n), complex)
-        lib().oracle_hk_point(self.h, dptr(_v3(kfrac)), dptr(_v3(E)), dptr(H0.view(float).reshape(-1)),
+        self.L.oracle_hk_point(self.h, dptr(_v3(kfrac)), dptr(_v3(E)), dptr(H0.view(float).reshape(-1)),
                               dptr(HE.view(float).reshape(-1)))
         return H0, HE
 
@@ -163,13 +166,13 @@ class Oracle:
         H0 = np.zeros((n, n), complex)
         D = np.zeros((3, n, n), complex)
         dH = np.zeros((3, n, n), complex)
-        lib().oracle_hk_full_point(self.h, dptr(_v3(kfrac)), dptr(H0.view(float).reshape(-1)),
+        self.L.oracle_hk_full_point(self.h, dptr(_v3(kfrac)), dptr(H0.view(float).reshape(-1)),
                                    dptr(D.view(float).reshape(-1)), dptr(dH.view(float).reshape(-1)))
         return H0, D, dH
 
     def hk_mesh(self, A=(0, 0, 0), E=(0, 0, 0)):
         out = np.zeros((self.num_k, 2, self.n, self.n), complex)
-        lib().oracle_hk_mesh(self.h, dptr(_v3(A)), dptr(_v3(E)), dptr(out.view(float).reshape(-1)))
+        self.L.oracle_hk_mesh(self.h, dptr(_v3(A)), dptr(_v3(E)), dptr(out.view(float).reshape(-1)))
         return out
 
     def new_state(self):
@@ -177,23 +180,23 @@ class Oracle:
 
     def rhs(self, state: np.ndarray, t: float) -> np.ndarray:
         out = np.zeros_like(state)
-        lib().oracle_rhs(self.h, dptr(state.view(float)), dptr(out.view(float)), t)
+        self.L.oracle_rhs(self.h, dptr(state.view(float)), dptr(out.view(float)), t)
         return out
 
     def initial_state(self, mu: float) -> np.ndarray:
         st = self.new_state()
-        lib().oracle_initial_state(self.h, mu, dptr(st.view(float)))
+        self.L.oracle_initial_state(self.h, mu, dptr(st.view(float)))
         return st
 
     def fermi_level(self, occupied_bands: int = 0, below_gap: bool = False) -> float:
         mu = C.c_double()
-        if lib().oracle_fermi_level(self.h, occupied_bands, int(below_gap), C.byref(mu)):
-            raise RuntimeError(lib().oracle_last_error().decode())
+        if self.L.oracle_fermi_level(self.h, occupied_bands, int(below_gap), C.byref(mu)):
+            raise RuntimeError(self.L.oracle_last_error().decode())
         return mu.value
 
     def observe_j(self, state: np.ndarray) -> np.ndarray:
         j = np.zeros(3)
-        lib().oracle_observe_j(self.h, dptr(state.view(float)), dptr(j))
+        self.L.oracle_observe_j(self.h, dptr(state.view(float)), dptr(j))
         return j
 
     def integrate(self, state: np.ndarray, output_count: int, replay: StepLog | None = None,
@@ -203,16 +206,21 @@ class Oracle:
         j = np.zeros((output_count, 3))
         log = StepLog(log_capacity)
         rhs = C.c_long()
-        steps = lib().oracle_integrate(self.h, dptr(state.view(float)), output_count, dptr(t),
+        steps = self.L.oracle_integrate(self.h, dptr(state.view(float)), output_count, dptr(t),
                                        dptr(j.reshape(-1)), C.byref(log.c),
                                        C.byref(replay.c) if replay is not None else None, C.byref(rhs))
         if steps < 0:
-            raise RuntimeError("oracle_integrate: " + lib().oracle_last_error().decode())
+            raise RuntimeError("oracle_integrate: " + self.L.oracle_last_error().decode())
         return {"t": t, "j": j, "steps": int(steps), "rhs_calls": int(rhs.value), "log": log}
+
+    @property
+    def num_threads(self) -> int:
+        """OpenMP threads of the oracle's parallel loops"""
+        return self.L.oracle_num_threads()
 
     def bench_attempts(self, state: np.ndarray, t: float, dt: float, count: int) -> float:
         """wall seconds of `count` dopri5 attempts (6 RHS each) at fixed (t, dt)."""
-        return lib().oracle_bench_attempts(self.h, dptr(state.view(float)), t, dt, count)
+        return self.L.oracle_bench_attempts(self.h, dptr(state.view(float)), t, dt, count)
 
     # reference layout helpers
     def rho(self, state: np.ndarray) -> np.ndarray:

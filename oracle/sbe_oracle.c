@@ -1070,6 +1070,8 @@ done:
  * attempt: 6 RHS evaluations plus stage algebra and error norm; the state is
  * not advanced).  Used as the CPU baseline of the benchmark. Returns the
  * elapsed wall time in seconds. */
+int oracle_num_threads(void) { return omp_get_max_threads(); }
+
 double oracle_bench_attempts(oracle_ctx *c, const double *state_d, double t, double dt, int count)
 {
     dp_work w;

@@ -73,6 +73,9 @@ long oracle_integrate(oracle_ctx *c, double *state, int output_count, double *t_
                       double *j_out, sbe_steplog *log, const sbe_steplog *replay,
                       long *rhs_calls);
 
+/* OpenMP threads the oracle's parallel loops use */
+int oracle_num_threads(void);
+
 /* wall seconds of `count` dopri5 attempts at fixed (t, dt) (CPU baseline) */
 double oracle_bench_attempts(oracle_ctx *c, const double *state, double t, double dt, int count);
 

@@ -24,7 +24,27 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{SRC}"]
 SOURCES = ["phase_kernels.hip", "stage_kernels.hip", "rk_kernels.hip", "observer_kernels.hip", "sbe_api.cpp",
            "scan_api.cpp", "spectrum.cpp", "mlp_kernels.hip", "mlp_api.cpp"]
-HEADERS = ["device_common.hpp", "kernels.hpp"]
+
+
+def _local_includes(path: str, seen=None) -> list:
+    """every file reached from `path` through #include "..." lines (the
+    source's own headers and include/*.h), so that an edit of any of them
+    rebuilds the objects that use it"""
+    seen = set() if seen is None else seen
+    base = os.path.dirname(path)
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("#include") and '"' in line:
+                name = line.split('"')[1]
+                for root in (base, SRC, INCLUDE):
+                    cand = os.path.normpath(os.path.join(root, name))
+                    if os.path.exists(cand):
+                        if cand not in seen:
+                            seen.add(cand)
+                            _local_includes(cand, seen)
+                        break
+    return sorted(seen)
 
 
 def _newer(a: str, deps: list) -> bool:
@@ -36,8 +56,7 @@ def _newer(a: str, deps: list) -> bool:
 
 def _compile(src: str, force: bool) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    deps = [os.path.join(SRC, src)] + [os.path.join(SRC, h) for h in HEADERS] + \
-        [os.path.join(INCLUDE, "sbewh_hip.h"), os.path.join(INCLUDE, "sbewh_types.h"), os.path.abspath(__file__)]
+    deps = [os.path.join(SRC, src)] + _local_includes(os.path.join(SRC, src)) + [os.path.abspath(__file__)]
     if not force and _newer(obj, deps):
         return obj
     lang = ["-x", "hip"]

@@ -59,6 +59,27 @@ int sbe_pulse_E(const sbe_ctx *ctx, double t, double E[3])
     return SBE_OK;
 }
 
+int sbe_pulse_field(const sbe_subpulse *pulses, int num_pulses, double t, double E[3])
+{
+    if (!E) return SBE_ERR_ARG;
+    sbe_ctx tmp;
+    int rc = guarded(nullptr, [&] { set_pulses(&tmp, pulses, num_pulses); });
+    if (rc) return rc;
+    tmp.E_at(t, E);
+    return SBE_OK;
+}
+
+int sbe_pulse_window(const sbe_subpulse *pulses, int num_pulses, double *t0, double *t1)
+{
+    if (!t0 || !t1) return SBE_ERR_ARG;
+    sbe_ctx tmp;
+    int rc = guarded(nullptr, [&] { set_pulses(&tmp, pulses, num_pulses); });
+    if (rc) return rc;
+    *t0 = tmp.t_start();
+    *t1 = tmp.t_end();
+    return SBE_OK;
+}
+
 int sbe_fermi_level(sbe_ctx *ctx, int occupied_bands, int below_gap, double *mu)
 {
     if (!ctx || !mu) return SBE_ERR_ARG;
