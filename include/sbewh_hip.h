@@ -8,7 +8,9 @@
  *   [A_x, A_y, A_z, 5 zero pads][rho_k0 (n*n, row-major, zero padded to P)][rho_k1]...
  * with P = 8 * ceil(n*n / 8) and k in Fourier-block order
  * (code/src/WhFourierTransform.cpp:137-149, 303-312).  Length
- * 8 + P * N_k complex entries (sbe_state_size).
+ * 8 + P * N_k complex entries (sbe_state_size).  A context created with
+ * SBE_FLAG_GPU_LAYOUT uses the reference's GPU layout instead (P = n*n, the
+ * same k order; code/src/CuPropagator.cu:50-52,111-114) in every entry point.
  *
  * Every function returns 0 on success and a negative code on error; the
  * message is available from sbe_last_error(ctx) (or sbe_last_error(NULL) for
@@ -57,6 +59,13 @@ int sbe_set_pulses(sbe_ctx *ctx, const sbe_subpulse *pulses, int num_pulses);
 
 long sbe_state_size(const sbe_ctx *ctx);   /* complex entries */
 long sbe_num_k(const sbe_ctx *ctx);
+/* complex stride P between consecutive rho_k of the context's state layout */
+int sbe_matrix_stride(const sbe_ctx *ctx);
+/* grid[k][3] = integer grid indices (i_0, i_1, i_2), 0 <= i_d < Nk_d, of the
+ * state's k-th density matrix (fractional k = i_d / Nk_d + kGlobalShift):
+ * the Fourier-block order of code/src/WhFourierTransform.cpp:137-149,303-312
+ * (getFineMeshFracShifts / getFFTfracShift), the same in both layouts. */
+int sbe_k_index_map(const sbe_ctx *ctx, int *grid);
 int sbe_get_nf(const sbe_ctx *ctx, int nf[3]);
 /* pulse window [t0, t1] (code/src/pulse/Pulse3D.cpp:42-62) */
 int sbe_time_window(const sbe_ctx *ctx, double *t0, double *t1);

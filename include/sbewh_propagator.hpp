@@ -10,8 +10,12 @@
 //
 // CPUstate is std::vector<std::complex<double>> in the reference's layout
 // (code/src/StateContext.h:7-25), so the functor drops into the reference's
-// odeint call unchanged.  Errors throw sbewh::Error (the ABI itself never
-// throws).
+// odeint call unchanged.  A context created with SBE_FLAG_GPU_LAYOUT takes
+// states in the layout of the reference's GPU path instead (8 special slots,
+// then rho_k unpadded, interleaved doubles: GPUstate of
+// code/src/CuPropagator.h:34); the functor accepts any contiguous container
+// of complex<double> or double of the context's state size.  Errors throw
+// sbewh::Error (the ABI itself never throws).
 #pragma once
 
 #include <array>
@@ -25,6 +29,19 @@
 namespace sbewh {
 
 using CPUstate = std::vector<std::complex<double>>;
+// host mirror of the reference's GPUstate (interleaved re/im doubles)
+using InterleavedState = std::vector<double>;
+
+namespace detail {
+inline const double *as_doubles(const CPUstate &s) { return reinterpret_cast<const double *>(s.data()); }
+inline double *as_doubles(CPUstate &s) { return reinterpret_cast<double *>(s.data()); }
+inline const double *as_doubles(const InterleavedState &s) { return s.data(); }
+inline double *as_doubles(InterleavedState &s) { return s.data(); }
+inline size_t complex_count(const CPUstate &s) { return s.size(); }
+inline size_t complex_count(const InterleavedState &s) { return s.size() / 2; }
+inline void resize_complex(CPUstate &s, size_t n) { s.resize(n); }
+inline void resize_complex(InterleavedState &s, size_t n) { s.resize(2 * n); }
+}  // namespace detail
 
 struct Error : std::runtime_error {
     int code;
@@ -51,6 +68,14 @@ public:
     Context &operator=(const Context &) = delete;
     sbe_ctx *get() const { return ctx_; }
     long stateSize() const { return sbe_state_size(ctx_); }
+    int matrixStride() const { return sbe_matrix_stride(ctx_); }
+    // grid indices (i_0, i_1, i_2) of every density matrix in state order
+    std::vector<std::array<int, 3>> kIndexMap() const
+    {
+        std::vector<std::array<int, 3>> g((size_t)sbe_num_k(ctx_));
+        check(sbe_k_index_map(ctx_, g.empty() ? nullptr : g[0].data()), ctx_);
+        return g;
+    }
     double fermiLevel(int occupiedBands, bool belowGap = false) const
     {
         double mu = 0;
@@ -72,18 +97,22 @@ class PropagatorHIP {
 public:
     explicit PropagatorHIP(const Context &c) : c_(c) {}
     void reset() {}
-    CPUstate calcInitialState(double fermiLevel) const
+    template <class State = CPUstate>
+    State calcInitialState(double fermiLevel) const
     {
-        CPUstate s((size_t)c_.stateSize());
-        check(sbe_initial_state(c_.get(), fermiLevel, reinterpret_cast<double *>(s.data())), c_.get());
+        State s;
+        detail::resize_complex(s, (size_t)c_.stateSize());
+        check(sbe_initial_state(c_.get(), fermiLevel, detail::as_doubles(s)), c_.get());
         return s;
     }
-    void operator()(const CPUstate &state, CPUstate &dsdt, const double t) const
+    // State: CPUstate or InterleavedState, in the context's layout
+    template <class State>
+    void operator()(const State &state, State &dsdt, const double t) const
     {
-        dsdt.resize(state.size());
-        check(sbe_rhs(c_.get(), reinterpret_cast<const double *>(state.data()), reinterpret_cast<double *>(dsdt.data()),
-                      t),
-              c_.get());
+        if ((long)detail::complex_count(state) != c_.stateSize())
+            throw Error(SBE_ERR_ARG, "sbewh: state size does not match the context's layout");
+        detail::resize_complex(dsdt, detail::complex_count(state));
+        check(sbe_rhs(c_.get(), detail::as_doubles(state), detail::as_doubles(dsdt), t), c_.get());
     }
 };
 
@@ -100,10 +129,11 @@ public:
         t.clear();
         j.clear();
     }
-    void operator()(const CPUstate &state, double time)
+    template <class State>
+    void operator()(const State &state, double time)
     {
         std::array<double, 3> v{};
-        check(sbe_observe(c_.get(), reinterpret_cast<const double *>(state.data()), v.data()), c_.get());
+        check(sbe_observe(c_.get(), detail::as_doubles(state), v.data()), c_.get());
         t.push_back(time);
         j.push_back(v);
     }
@@ -112,12 +142,12 @@ public:
 // integrate_const over the pulse window with outputCount outputs, state
 // resident on the GPU for the whole run; fills the observer; returns the
 // number of accepted steps.
-inline long integrate_const(const Context &c, CPUstate &state, int outputCount, ObserverHIP &obs,
-                            long *rhsCalls = nullptr)
+template <class State>
+long integrate_const(const Context &c, State &state, int outputCount, ObserverHIP &obs, long *rhsCalls = nullptr)
 {
     std::vector<double> t(outputCount), j(3 * (size_t)outputCount);
-    long steps = sbe_integrate(c.get(), reinterpret_cast<double *>(state.data()), outputCount, t.data(), j.data(),
-                               nullptr, nullptr, rhsCalls);
+    long steps = sbe_integrate(c.get(), detail::as_doubles(state), outputCount, t.data(), j.data(), nullptr, nullptr,
+                               rhsCalls);
     check(steps, c.get());
     obs.reset();
     for (int i = 0; i < outputCount; i++) {

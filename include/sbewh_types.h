@@ -66,7 +66,16 @@ enum {
     /* start every Jacobi diagonalisation of the RHS from the identity (the
      * reference's behaviour) instead of from the previous RHS call's
      * eigenvectors at the same k-point */
-    SBE_FLAG_COLD_JACOBI = 1
+    SBE_FLAG_COLD_JACOBI = 1,
+    /* state vectors of this context in the reference's GPU layout
+     * (CuPropagator, code/src/CuPropagator.cu:50-52,111-114): the 8 special
+     * complex slots, then rho_k unpadded (n*n complex each, row-major), k in
+     * the same Fourier-block order; length 8 + N_k * n * n complex
+     * (2 * that many interleaved doubles, the thrust::device_vector<double>
+     * GPUstate).  Without the flag: the CPU layout with each rho_k padded to
+     * 8 * ceil(n*n / 8) complex (code/src/StateContext.h:7-25).  Both layouts
+     * are identical when n*n is a multiple of 8. */
+    SBE_FLAG_GPU_LAYOUT = 2
 };
 
 enum sbe_pulse_type {

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libsbewh_hip.so")
 EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_set_pulses", "sbe_state_size",
            "sbe_num_k", "sbe_get_nf", "sbe_time_window", "sbe_pulse_E", "sbe_pulse_field", "sbe_pulse_window",
            "sbe_fermi_level", "sbe_initial_state",
-           "sbe_rhs", "sbe_rhs_device", "sbe_observe", "sbe_integrate", "sbe_hk_mesh", "sbe_jacobi_batch",
+           "sbe_matrix_stride", "sbe_k_index_map", "sbe_rhs", "sbe_rhs_device", "sbe_observe", "sbe_integrate", "sbe_hk_mesh", "sbe_jacobi_batch",
            "sbe_bench_steps", "sbe_stage_tuning", "sbe_hhg_spectrum", "sbe_harmonic_yields", "sbe_scan_create",
            "sbe_scan_destroy", "sbe_scan_last_error", "sbe_scan_run", "sbe_scan_state_size", "sbe_scan_final_state",
            "sbe_scan_tuning")
@@ -54,6 +54,8 @@ def load():
     L.sbe_num_k.restype = l
     L.sbe_num_k.argtypes = [vp]
     L.sbe_get_nf.argtypes = [vp, P(C.c_int)]
+    L.sbe_matrix_stride.argtypes = [vp]
+    L.sbe_k_index_map.argtypes = [vp, P(C.c_int)]
     L.sbe_time_window.argtypes = [vp, dp, dp]
     L.sbe_pulse_E.argtypes = [vp, d, dp]
     L.sbe_pulse_field.argtypes = [P(SbeSubpulse), i, d, dp]

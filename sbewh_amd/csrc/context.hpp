@@ -139,6 +139,7 @@ struct sbe_ctx {
     std::vector<SubPulse> pulses;
     PhaseGeom geom{};
     KOrderH korder{};
+    std::vector<int> kgrid;    // [k][3] grid indices of state k (Fourier-block order)
     int obs_ops = 0;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -587,6 +588,7 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
     if (m->num_cells < 1 || !m->cells || !m->H || !m->D) fail(SBE_ERR_ARG, "empty model");
     if (g->dim < 1 || g->dim > 3) fail(SBE_ERR_ARG, "dim must be 1, 2 or 3");
     if (prm->sweep_count < 0 || prm->sweep_count > 50) fail(SBE_ERR_ARG, "sweep_count must be in [0, 50]");
+    if (prm->flags & ~(SBE_FLAG_COLD_JACOBI | SBE_FLAG_GPU_LAYOUT)) fail(SBE_ERR_ARG, "unknown flags 0x%x", prm->flags);
     const int n = m->num_wann, nn = n * n;
     c->n = n;
     c->nn = nn;
@@ -650,7 +652,8 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
         c->korder.Nrep[d] = Nk / Nf[d];
     }
     c->nk = (long)G.Nk[0] * G.Nk[1] * G.Nk[2];
-    c->P = 8 * ((nn + 7) / 8);
+    // reference CPU layout: rho_k padded to a multiple of 8 complex; GPU layout: unpadded
+    c->P = (prm->flags & SBE_FLAG_GPU_LAYOUT) ? nn : 8 * ((nn + 7) / 8);
     c->S = 8 + (long)c->P * c->nk;
     // kGlobalShift -> a.u. (code/src/StateContext.cpp:3-9)
     {
@@ -694,6 +697,9 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
                 cr = (int)(blk % o.Nrep[2]);
             long i0 = a * o.Nrep[0] + ar, i1 = bb * o.Nrep[1] + br, i2 = cc * o.Nrep[2] + cr;
             kmap[(i0 * G.Nk[1] + i1) * G.Nk[2] + i2] = (int)k;
+            c->kgrid.push_back((int)i0);
+            c->kgrid.push_back((int)i1);
+            c->kgrid.push_back((int)i2);
         }
     }
     if (c->nk > 0x7fffffffL) fail(SBE_ERR_ARG, "too many k-points");

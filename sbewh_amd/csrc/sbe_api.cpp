@@ -36,6 +36,14 @@ int sbe_set_pulses(sbe_ctx *ctx, const sbe_subpulse *pulses, int num_pulses)
 
 long sbe_state_size(const sbe_ctx *ctx) { return ctx ? ctx->S : SBE_ERR_ARG; }
 long sbe_num_k(const sbe_ctx *ctx) { return ctx ? ctx->nk : SBE_ERR_ARG; }
+int sbe_matrix_stride(const sbe_ctx *ctx) { return ctx ? ctx->P : SBE_ERR_ARG; }
+
+int sbe_k_index_map(const sbe_ctx *ctx, int *grid)
+{
+    if (!ctx || !grid) return SBE_ERR_ARG;
+    std::memcpy(grid, ctx->kgrid.data(), ctx->kgrid.size() * sizeof(int));
+    return SBE_OK;
+}
 
 int sbe_get_nf(const sbe_ctx *ctx, int nf[3])
 {

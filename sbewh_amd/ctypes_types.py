@@ -114,7 +114,8 @@ class Inputs:
         self.params.eps_abs = cfg.eps_abs
         self.params.eps_rel = cfg.eps_rel
         self.params.sweep_count = cfg.sweeps
-        self.params.flags = 1 if getattr(cfg, "cold_jacobi", False) else 0
+        self.params.flags = (1 if getattr(cfg, "cold_jacobi", False) else 0) | \
+            (2 if getattr(cfg, "gpu_layout", False) else 0)
         self.pulses = pulse_array(cfg.pulses)
         self.num_pulses = len(cfg.pulses)
 

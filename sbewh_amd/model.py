@@ -120,6 +120,7 @@ class Config:
     output_count: int = 1024
     k_global_shift: tuple = (0.0, 0.0, 0.0)
     cold_jacobi: bool = False   # start each RHS's Jacobi from the identity
+    gpu_layout: bool = False    # state vectors in the reference's GPU layout (rho_k unpadded)
 
     @property
     def num_k(self) -> int:

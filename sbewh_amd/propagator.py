@@ -40,7 +40,7 @@ class SBEContext:
         self.n = cfg.model.num_wann
         self.state_size = L.sbe_state_size(h)
         self.num_k = L.sbe_num_k(h)
-        self.P = 8 * ((self.n * self.n + 7) // 8)
+        self.P = L.sbe_matrix_stride(h)
 
     def close(self):
         if getattr(self, "h", None):
@@ -139,6 +139,12 @@ class SBEContext:
                                      C.byref(rhs), C.byref(sms) if kernel_timing else None,
                                      C.byref(pms) if kernel_timing else None), self.h)
         return {"ms": ms.value, "rhs": rhs.value, "stage_ms": sms.value, "phase_ms": pms.value}
+
+    def k_index_map(self) -> np.ndarray:
+        """(N_k, 3) integer grid indices of the state's density matrices"""
+        g = np.zeros((self.num_k, 3), np.int32)
+        check(load().sbe_k_index_map(self.h, g.ctypes.data_as(C.POINTER(C.c_int))), self.h)
+        return g
 
     def stage_tuning(self) -> dict:
         lr, cnt = C.c_int(), C.c_int()
