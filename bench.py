@@ -65,12 +65,9 @@ def pmc_traffic(path: str):
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """device-wide synchronisation through the library's own HIP runtime"""
+    from sbewh_amd import hiprt
+    hiprt.synchronize()
 
 
 def physical_cores():
@@ -287,14 +284,15 @@ def main():
                     help="only run a device copy of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
-        n = args.copy_calibration * (1 << 20) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
+        import numpy as np
+        from sbewh_amd import hiprt
+        nbytes = args.copy_calibration * (1 << 20)
+        a = hiprt.DeviceBuffer.from_array(np.ones(nbytes // 8))
+        b = hiprt.DeviceBuffer(nbytes)
         for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}), flush=True)
+            b.copy_from(a, nbytes)
+        hiprt.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding the k-points over "

@@ -116,6 +116,59 @@ int sbe_observe(sbe_ctx *ctx, const double *state, double j[3]);
 long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out, double *j_out,
                    sbe_steplog *log, const sbe_steplog *replay, long *rhs_calls);
 
+/* ---- full observer ------------------------------------------------------
+ * ObserverCPU with k-space regions, occupations and the region-averaged
+ * density matrix (code/src/ObserverCPU.h:43, body code/src/ObserverCPU.cpp:
+ * 137-260, normalizeExpectationValues :293-308; regions from
+ * code/src/ObserverBase.cpp:3-17).  At each recorded output time and for
+ * every region sample r (sbe_region, weights w_r(k)):
+ *   weight_sum[r] = sum_k w_r(k)
+ *   j[r][d]       = sum_k w_r(k) Re[i Tr(D_d [H0, rho_k]) - Tr(dH0/dk_d rho_k)] / weight_sum[r]
+ *   occ_ham[r][b] = sum_k w_r(k) o_b(k) / weight_sum[r],  occ_wan[r][b] = sum_k w_r(k) Re rho_k,bb / weight_sum[r]
+ *   wan[r]        = sum_k w_r(k) rho_k / weight_sum[r]  (records with index % density_stride == 0)
+ *   occ_ham_min[b], occ_ham_max[b] = min / max over k of o_b(k)
+ * with H0, D, dH0/dk at k + A, o_b(k) the Hamiltonian occupation
+ * <v_b|rho_k|v_b> of the b-th eigenvector of H0(k + A) in ascending energy,
+ * smeared over energy, o_b <- sum_q o_q exp(-((e_b - e_q)/w)^2) / sum_q exp(..)
+ * when smearing_width w > 0 (the reference's occupationSmearingWidth; w <= 0:
+ * unsmeared).  Bands are numbered in ascending energy, the order of the
+ * reference's default diagonaliser (LAPACK zheev). */
+typedef struct sbe_observer sbe_observer;
+
+/* Collected records, caller-allocated arrays (any may be NULL); count =
+ * records, R = regions, n = bands, density records counted separately:
+ * t[count], E[count][3] (E(t), zero beyond dim), A[count][3],
+ * weight_sum[count][R], j[count][R][3], occ_ham[count][R][n],
+ * occ_wan[count][R][n], occ_ham_min[count][n], occ_ham_max[count][n],
+ * wan[density_count][R][n][n] (complex, interleaved), density_index[density_count]
+ * (the record index of each density record; ExpectationValues_t::tDensIndices). */
+typedef struct sbe_observations {
+    double *t, *E, *A, *weight_sum, *j, *occ_ham, *occ_wan, *occ_ham_min, *occ_ham_max, *wan;
+    int *density_index;
+} sbe_observations;
+
+/* regions[num_regions]: the region samples in output order (the reference puts
+ * the whole-zone region "expectationValues" first, then the KspaceRegions
+ * entries by name, each expanded over its sample grid).  density_stride 0: no
+ * density matrices.  capacity: records the observer can hold. */
+int sbe_observer_create(sbe_ctx *ctx, const sbe_region *regions, int num_regions, double smearing_width,
+                        int density_stride, int capacity, sbe_observer **out);
+void sbe_observer_destroy(sbe_observer *obs);
+/* forget the records (ObserverCPU::reset, code/src/ObserverCPU.cpp:131-136) */
+int sbe_observer_reset(sbe_observer *obs);
+/* ObserverCPU::operator()(const CPUstate&, double t) on a host state */
+int sbe_observer_record(sbe_observer *obs, const double *state, double t);
+/* the same on a device state, enqueued on `stream` (NULL: the context's) */
+int sbe_observer_record_device(sbe_observer *obs, const double *d_state, double t, void *stream);
+/* records so far (return value) and how many of them hold density matrices */
+int sbe_observer_count(const sbe_observer *obs, int *density_count);
+/* copy the records out (collectExpectationValues, code/src/ObserverBase.cpp:66-112) */
+int sbe_observer_results(sbe_observer *obs, sbe_observations *out);
+/* sbe_integrate with the full observer recording every output time (its
+ * capacity must cover output_count more records) */
+long sbe_integrate_observed(sbe_ctx *ctx, double *state, int output_count, sbe_observer *obs, double *t_out,
+                            double *j_out, sbe_steplog *log, const sbe_steplog *replay, long *rhs_calls);
+
 /* Kernel-level entry points (used by tests and the benchmark). */
 
 /* H0(k + A) and H_E(k + A) for every k, out[k][2][n*n] complex (the

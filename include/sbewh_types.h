@@ -112,6 +112,21 @@ typedef struct sbe_instance {
     double relaxation_F2;     /* 1/T2; <= 0 disables dephasing */
 } sbe_instance;
 
+/* One sample point of a k-space region of the observer (ObserverRegion_t,
+ * code/src/ObserverBase.h:33-37, built by code/src/ObserverBase.cpp:3-17
+ * from the KspaceRegions input, code/src/io/KspaceRegionParameter.cpp:78-164).
+ * The weight of state k-point k in the region is
+ *   w = prod_{d<3} exp(-c_d^2 * 8 ln2 / fwhm^2),  c_d = p_d - round(p_d),
+ *   p = k_frac(k) + k_global_shift + (moving_frame ? 0 : frac(A)) - k,
+ * with frac(A)_d = A . a_d / (2 pi) (code/src/ObserverCPU.cpp:137-147,
+ * code/src/WhFourierTransform.cpp:46-52,163-174).  fwhm = INFINITY gives
+ * w = 1 (the default region "expectationValues"). */
+typedef struct sbe_region {
+    double k[3];              /* centre, fractional coordinates */
+    double fwhm;              /* full width at half maximum, fractional */
+    int moving_frame;         /* 1: 'M' (centre follows k), 0: 'F' (fixed in the k + A frame) */
+} sbe_region;
+
 /* Record of the adaptive controller: one entry per step attempt. */
 typedef struct sbe_steplog {
     int capacity;             /* entries the arrays can hold */

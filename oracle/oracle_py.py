@@ -12,7 +12,8 @@ import subprocess
 
 import numpy as np
 
-from sbewh_amd.ctypes_types import Inputs, SbeModel, SbeGrid, SbeParams, SbeSubpulse, SbeSteplog, StepLog, dptr
+from sbewh_amd.ctypes_types import (Inputs, SbeModel, SbeGrid, SbeParams, SbeRegion, SbeSubpulse, SbeSteplog,
+                                     StepLog, dptr, region_array)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "libsbe_oracle.so")
@@ -64,6 +65,7 @@ def lib(path: str = LIB_PATH):
         L.oracle_initial_state.argtypes = [vp, C.c_double, P(C.c_double)]
         L.oracle_fermi_level.argtypes = [vp, C.c_int, C.c_int, P(C.c_double)]
         L.oracle_observe_j.argtypes = [vp, P(C.c_double), P(C.c_double)]
+        L.oracle_observe_full.argtypes = [vp, P(C.c_double), P(SbeRegion), C.c_int, C.c_double] + [P(C.c_double)] * 7
         L.oracle_integrate.restype = C.c_long
         L.oracle_integrate.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_double), P(C.c_double),
                                        P(SbeSteplog), P(SbeSteplog), P(C.c_long)]
@@ -198,6 +200,22 @@ class Oracle:
         j = np.zeros(3)
         self.L.oracle_observe_j(self.h, dptr(state.view(float)), dptr(j))
         return j
+
+    def observe_full(self, state: np.ndarray, regions, smearing: float, density: bool = False) -> dict:
+        """ObserverCPU at one output time for the region samples `regions`
+        ((k[3], fwhm, moving) each): dict of weight_sum [R], j [R,3],
+        occ_ham [R,n], occ_wan [R,n], occ_min [n], occ_max [n], wan [R,n,n]"""
+        R, n = len(regions), self.n
+        out = {"weight_sum": np.zeros(R), "j": np.zeros((R, 3)), "occ_ham": np.zeros((R, n)),
+               "occ_wan": np.zeros((R, n)), "occ_min": np.zeros(n), "occ_max": np.zeros(n)}
+        wan = np.zeros((R, n, n), complex) if density else None
+        self.L.oracle_observe_full(self.h, dptr(state.view(float)), region_array(regions), R, smearing,
+                                   dptr(out["weight_sum"]), dptr(out["j"].reshape(-1)), dptr(out["occ_ham"].reshape(-1)),
+                                   dptr(out["occ_wan"].reshape(-1)), dptr(out["occ_min"]), dptr(out["occ_max"]),
+                                   dptr(wan.view(float).reshape(-1)) if density else None)
+        if density:
+            out["wan"] = wan
+        return out
 
     def integrate(self, state: np.ndarray, output_count: int, replay: StepLog | None = None,
                   log_capacity: int = 400000):

@@ -862,6 +862,165 @@ int oracle_observe_j(oracle_ctx *c, const double *state_d, double jout[3])
 }
 
 /* ------------------------------------------------------------------------ */
+/* Full observer at one output time: ObserverCPU::operator()
+ * (code/src/ObserverCPU.cpp:137-260) and normalizeExpectationValues (:293-308),
+ * regions from code/src/ObserverBase.cpp:3-17.  Per k: current j_k (as
+ * oracle_observe_j), eigen-decomposition of H0(k + A) (cyclic Jacobi, sweep
+ * count of the parameters), Hamiltonian occupations <v_b|rho|v_b> smeared over
+ * energy with weights exp(-((e_a - e_b)/w)^2) normalised over b (w <= 0: no
+ * smearing), Wannier occupations Re rho_bb, and the density matrix; each
+ * summed with the region weight (calcWeight, :137-147) and divided by the
+ * weight sum; min / max over k of the smeared occupations.  Bands are
+ * numbered in ascending energy (the order LAPACK zheev, the reference's
+ * default diagonaliser, returns; the Jacobi's own order is not sorted). */
+static double region_weight(const sbe_region *r, const double kgrid[3], const double off[3], const double negA[3])
+{
+    double inv = isinf(r->fwhm) ? 0.0 : 8 * log(2.0) / (r->fwhm * r->fwhm);
+    double w = 1;
+    for (int d = 0; d < 3; d++) {
+        double pos = kgrid[d] + off[d];
+        if (r->moving_frame) pos += negA[d];
+        pos -= r->k[d];
+        double cp = pos - round(pos);
+        w *= exp(-cp * cp * inv);
+    }
+    return w;
+}
+
+int oracle_observe_full(oracle_ctx *c, const double *state_d, const sbe_region *reg, int R, double smearing,
+                        double *weight_sum, double *j_out, double *occ_ham, double *occ_wan, double *occ_min,
+                        double *occ_max, double *wan)
+{
+    const cplx *state = (const cplx *)state_d;
+    const int n = c->n, nn = n * n, dim = c->dim;
+    double A[3], kOff[3], off[3], mA[3], negA[3];
+    for (int d = 0; d < 3; d++) {
+        A[d] = creal(state[d]);
+        kOff[d] = c->kshift_au[d] + A[d];
+        mA[d] = -A[d];
+    }
+    to_frac_k(c->lat, kOff, off);
+    to_frac_k(c->lat, mA, negA);
+    /* per k-point quantities first (in parallel), then the region sums in k order */
+    const int Q = 3 + 2 * n;
+    double *X = calloc((size_t)c->nk * Q, sizeof(double));
+    double *W = malloc(sizeof(double) * (size_t)c->nk * (R > 0 ? R : 1));
+#pragma omp parallel
+    {
+        cplx *H0 = malloc(sizeof(cplx) * nn), *D = malloc(sizeof(cplx) * 3 * nn), *dH = malloc(sizeof(cplx) * 3 * nn);
+        cplx *T = malloc(sizeof(cplx) * nn), *U = malloc(sizeof(cplx) * nn);
+        double *eig = malloc(sizeof(double) * n), *occ = malloc(sizeof(double) * n);
+        int *ord = malloc(sizeof(int) * n);
+#pragma omp for schedule(static)
+        for (long k = 0; k < c->nk; k++) {
+            int ii[3];
+            double kf[3], kg[3];
+            k_index(c, k, ii);
+            for (int d = 0; d < 3; d++) {
+                kg[d] = (double)ii[d] / c->Nk[d];
+                kf[d] = kg[d] + off[d];
+            }
+            for (int r = 0; r < R; r++) W[(size_t)k * R + r] = region_weight(reg + r, kg, off, negA);
+            oracle_hk_full_point(c, kf, (double *)H0, (double *)D, (double *)dH);
+            const cplx *rho = state + c->nspecial + (size_t)c->P * k;
+            double *x = X + (size_t)k * Q;
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    cplx s = 0;
+                    for (int i = 0; i < n; i++) s += H0[a * n + i] * rho[i * n + b] - H0[i * n + b] * rho[a * n + i];
+                    T[a * n + b] = s;
+                }
+            for (int d = 0; d < dim; d++) {
+                cplx jk = 0;
+                for (int a = 0; a < n; a++)
+                    for (int b = 0; b < n; b++)
+                        jk += I * D[d * nn + a * n + b] * T[b * n + a] - dH[d * nn + a * n + b] * rho[b * n + a];
+                x[d] = creal(jk);
+            }
+            /* eigen-decomposition of H0(k + A): rows of U are conjugated eigenvectors */
+            jacobi_work(n, c->prm.sweep_count, H0, eig, U);
+            /* ascending energy (stable for ties) */
+            for (int i = 0; i < n; i++) ord[i] = i;
+            for (int i = 1; i < n; i++)
+                for (int m = i; m > 0 && eig[ord[m - 1]] > eig[ord[m]]; m--) {
+                    int tmp = ord[m];
+                    ord[m] = ord[m - 1];
+                    ord[m - 1] = tmp;
+                }
+            /* raw occupations (U rho U^dagger)_bb, code/src/ObserverCPU.cpp:228-233 */
+            for (int s = 0; s < n; s++) {
+                int bi = ord[s];
+                cplx sum = 0;
+                for (int a = 0; a < n; a++)
+                    for (int b = 0; b < n; b++) sum += conj(U[bi * n + b]) * rho[a * n + b] * U[bi * n + a];
+                occ[s] = creal(sum);
+            }
+            for (int s = 0; s < n; s++) {
+                double v;
+                if (smearing > 0) {
+                    double wsum = 0, acc = 0;
+                    for (int q = 0; q < n; q++) {
+                        double dE = (eig[ord[s]] - eig[ord[q]]) / smearing;
+                        double w = exp(-dE * dE);
+                        wsum += w;
+                        acc += occ[q] * w;
+                    }
+                    v = acc / wsum;
+                } else {
+                    v = occ[s];
+                }
+                x[3 + s] = v;
+                x[3 + n + s] = creal(rho[s * (n + 1)]);
+            }
+        }
+        free(H0);
+        free(D);
+        free(dH);
+        free(T);
+        free(U);
+        free(eig);
+        free(occ);
+        free(ord);
+    }
+    for (int s = 0; s < n; s++) {
+        occ_min[s] = INFINITY;
+        occ_max[s] = -INFINITY;
+    }
+    for (long k = 0; k < c->nk; k++)
+        for (int s = 0; s < n; s++) {
+            double v = X[(size_t)k * Q + 3 + s];
+            if (v < occ_min[s]) occ_min[s] = v;
+            if (v > occ_max[s]) occ_max[s] = v;
+        }
+    for (int r = 0; r < R; r++) {
+        double ws = 0, acc[3 + 2 * 16] = {0};
+        cplx *wr = wan ? (cplx *)wan + (size_t)r * nn : NULL;
+        if (wr)
+            for (int e = 0; e < nn; e++) wr[e] = 0;
+        for (long k = 0; k < c->nk; k++) {
+            double w = W[(size_t)k * R + r];
+            ws += w;
+            for (int q = 0; q < Q; q++) acc[q] += X[(size_t)k * Q + q] * w;
+            if (wr) {
+                const cplx *rho = state + c->nspecial + (size_t)c->P * k;
+                for (int e = 0; e < nn; e++) wr[e] += rho[e] * w;
+            }
+        }
+        weight_sum[r] = ws;
+        for (int d = 0; d < 3; d++) j_out[3 * r + d] = acc[d] / ws;
+        for (int s = 0; s < n; s++) {
+            occ_ham[(size_t)r * n + s] = acc[3 + s] / ws;
+            occ_wan[(size_t)r * n + s] = acc[3 + n + s] / ws;
+        }
+        if (wr)
+            for (int e = 0; e < nn; e++) wr[e] /= ws;
+    }
+    free(X);
+    free(W);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Dormand-Prince 5(4) with Boost.Odeint's controller semantics, restated from
  * the library's published algorithm (Boost.Odeint >= 1.53; exact version not
  * pinned by the reference, code/CMakeLists.txt:28):

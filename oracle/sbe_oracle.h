@@ -63,6 +63,12 @@ int oracle_rhs(oracle_ctx *c, const double *state, double *dsdt, double t);
 int oracle_initial_state(oracle_ctx *c, double mu, double *state);
 int oracle_fermi_level(oracle_ctx *c, int occupied_bands, int below_gap, double *mu);
 int oracle_observe_j(oracle_ctx *c, const double *state, double j[3]);
+/* full observer (regions, occupations, density): per region r
+ * weight_sum[r], j[r][3], occ_ham[r][n], occ_wan[r][n], wan[r][n*n] complex
+ * (optional), and occ_min[n], occ_max[n] over k */
+int oracle_observe_full(oracle_ctx *c, const double *state, const sbe_region *regions, int num_regions,
+                        double smearing, double *weight_sum, double *j, double *occ_ham, double *occ_wan,
+                        double *occ_min, double *occ_max, double *wan);
 
 /* integrate_const( controlled dopri5, rhs, state, t0, t1 + dt_out/2, dt_out, observer )
  * over the pulse window with output_count outputs.  j_out: [output_count][3],

@@ -7,7 +7,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .ctypes_types import SbeModel, SbeGrid, SbeParams, SbeSubpulse, SbeSteplog, SbeInstance
+from .ctypes_types import (SbeModel, SbeGrid, SbeParams, SbeSubpulse, SbeSteplog, SbeInstance, SbeRegion,
+                           SbeObservations)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libsbewh_hip.so")
@@ -19,7 +20,9 @@ EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_se
            "sbe_matrix_stride", "sbe_k_index_map", "sbe_rhs", "sbe_rhs_device", "sbe_observe", "sbe_integrate", "sbe_hk_mesh", "sbe_jacobi_batch",
            "sbe_bench_steps", "sbe_stage_tuning", "sbe_hhg_spectrum", "sbe_harmonic_yields", "sbe_scan_create",
            "sbe_scan_destroy", "sbe_scan_last_error", "sbe_scan_run", "sbe_scan_state_size", "sbe_scan_final_state",
-           "sbe_scan_tuning")
+           "sbe_scan_tuning", "sbe_observer_create", "sbe_observer_destroy", "sbe_observer_reset",
+           "sbe_observer_record", "sbe_observer_record_device", "sbe_observer_count", "sbe_observer_results",
+           "sbe_integrate_observed")
 # every symbol include/sbewh_surrogate.h declares
 SURROGATE_EXPORTS = ("sbe_mlp_create", "sbe_mlp_destroy", "sbe_mlp_last_error", "sbe_mlp_param_count",
                      "sbe_mlp_get_params", "sbe_mlp_set_params", "sbe_mlp_forward", "sbe_mlp_gradient",
@@ -83,6 +86,15 @@ def load():
     L.sbe_scan_state_size.argtypes = [vp, i]
     L.sbe_scan_final_state.argtypes = [vp, i, dp]
     L.sbe_scan_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), dp]
+    L.sbe_observer_create.argtypes = [vp, P(SbeRegion), i, d, i, i, P(vp)]
+    L.sbe_observer_destroy.argtypes = [vp]
+    L.sbe_observer_reset.argtypes = [vp]
+    L.sbe_observer_record.argtypes = [vp, dp, d]
+    L.sbe_observer_record_device.argtypes = [vp, vp, d, vp]
+    L.sbe_observer_count.argtypes = [vp, P(C.c_int)]
+    L.sbe_observer_results.argtypes = [vp, P(SbeObservations)]
+    L.sbe_integrate_observed.restype = l
+    L.sbe_integrate_observed.argtypes = [vp, dp, i, vp, dp, dp, P(SbeSteplog), P(SbeSteplog), P(C.c_long)]
     fp = P(C.c_float)
     L.sbe_mlp_create.argtypes = [i, i, P(C.c_int), C.c_ulonglong, P(vp)]
     L.sbe_mlp_destroy.argtypes = [vp]

@@ -197,6 +197,22 @@ struct sbe_ctx {
 
     static int default_lr(int N) { return N <= 4 ? 1 : 2; }
 
+    // Jacobi sweep bound of one diagonalisation.  A call that starts from the
+    // previous call's eigenvectors at the same k (warm) runs at most the
+    // configured sweep count; one that starts from the identity runs at most
+    // max(sweep count, COLD_SWEEPS), so that the first RHS of a run (and the
+    // ground state) begins the warm chain from a converged decomposition (16
+    // bands need 6 sweeps of the parallel ordering from the identity).  The
+    // iteration ends after the first sweep without a rotation, so a bound
+    // above what convergence needs changes nothing.  SBE_FLAG_COLD_JACOBI
+    // keeps the reference's fixed count on every call.
+    static constexpr int COLD_SWEEPS = 12;
+    int sweeps_for(int warm) const
+    {
+        if (warm || (prm.flags & SBE_FLAG_COLD_JACOBI)) return prm.sweep_count;
+        return std::max(prm.sweep_count, COLD_SWEEPS);
+    }
+
     // Time every compiled layout variant of the stage kernel on the current
     // inputs (3 timed launches after one warm-up each) and keep the fastest.
     // SBEWH_STAGE_LR in the environment forces a variant.
@@ -291,6 +307,7 @@ struct sbe_ctx {
         StageArgs a = stage_args(in, dxdt, E, ph, xout, err_e, dt);
         if (!lr) tune_stage(a, s);
         a.warm = (ucache_valid && !(prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
+        a.sweeps = sweeps_for(a.warm);
         HIPCHK(launch_rhs_stage(N, lr, a, s));
         if (a.dephase) ucache_valid = true;
         if (timing) HIPCHK(hipEventRecord(next_event(), s));
@@ -400,6 +417,7 @@ struct sbe_ctx {
         o.dim = dim;
         o.partial = partial;
         o.j_out = jdst;
+        o.pstride = 3;
         HIPCHK(launch_observe_j(o, s));
     }
 
@@ -451,6 +469,7 @@ struct sbe_ctx {
                 ss.st[q] = stage_args(ins[q], outs[q], E[q], last_set(q), q == 5 ? xn : nullptr,
                                       q == 5 ? err_e : nullptr, dt);
                 ss.st[q].warm = (ucache_valid && !(prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
+                ss.st[q].sweeps = sweeps_for(ss.st[q].warm);
                 if (ss.st[q].dephase) ucache_valid = true;
             }
             HIPCHK(launch_rhs_stage_set(N, lr, ss, s));

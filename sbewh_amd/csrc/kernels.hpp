@@ -244,11 +244,67 @@ struct ObsArgs {
     int n;
     int P;
     int dim;
-    double *partial;       // [nk][3]
+    double *partial;       // [nk][pstride], j_k in entries 0..2 of each row
     double *j_out;         // 3 doubles (normalised by nk)
+    int pstride;           // row stride of partial (3 for the current alone)
 };
 hipError_t launch_observe_j(const ObsArgs &a, hipStream_t s);
 // entry j owns blocks [block0[j], block0[j] + nk_j) (one block per k-point)
 hipError_t launch_observe_j_batched(const ObsArgs *d_args, const long *d_block0, int m, long blocks, hipStream_t s);
+// the per-k current alone: partial[k * pstride + d] = j_k,d (d < 3), no reduction
+hipError_t launch_observe_jk(const ObsArgs &a, hipStream_t s);
+
+// ---- full observer: occupations and k-space regions ----------------------
+// Per k-point Hamiltonian and Wannier occupations (stage_kernels.hip):
+// eigen-decomposition of H0(k + A) (op 0 of the observer partial phase sum),
+// occupations (U rho U^dagger)_bb in ascending energy, smeared over energy
+// with exp(-((e_a - e_b)/smear)^2) normalised over b when smear > 0;
+//   X[k Q + 3 + s] = smeared occupation of band s,  X[k Q + 3 + n + s] = Re rho_ss.
+struct OccArgs {
+    const double *x;       // state
+    PhaseLast ph;          // observer partial phase sum (ops == 1 + 2 dim)
+    int M;                 // (1 + 2 dim) n n
+    long nk;
+    int n;
+    int P;
+    int sweeps;
+    double smear;
+    double *X;
+    int Q;                 // row stride of X: 3 + 2 n
+};
+hipError_t launch_occupations(int N, int LR, const OccArgs &a, hipStream_t s);
+
+// Region-weighted sums over k (observer_kernels.hip), the reference's
+// ObserverCPU::calcWeight / operator() / normalizeExpectationValues
+// (code/src/ObserverCPU.cpp:137-147,149-260,293-308).  Quantity q of row k:
+// q < Q: X[k Q + q];  q == Q: 1 (the weight sum);  Q < q < Q + 1 + 2 n n: the
+// density matrix entries (interleaved doubles of rho_k) when `density`.
+struct RegionDev {
+    double k[3];
+    double inv_sigma2;     // 8 ln 2 / fwhm^2 (0: whole zone)
+    int moving;
+};
+struct RegionArgs {
+    const double *x;       // state (A in the special slots, rho_k at 8 + k P)
+    const double *X;       // [nk][Q]
+    const RegionDev *reg;  // [R]
+    int R;
+    int Q;
+    int n, P;
+    int density;
+    long nk;
+    KOrderH ko;            // k index -> grid indices
+    int Nk[3];
+    double shift[3];       // fractional k_global_shift
+    double lat[9];         // lattice rows a_d (bohr): frac(A)_d = A . a_d / 2 pi
+    int dim;
+    int chunks;            // k-chunks (grid.y)
+    long chunk;            // k-points per chunk
+    double *part;          // [R][chunks][Qtot] weighted partial sums
+    double *pmin, *pmax;   // [chunks][n] partial min / max over k of X[. + 3 + s]
+    double *rec;           // output record: A[3], [R][rstride] normalised (q == Q: raw weight sum), min[n], max[n]
+    int rstride;           // >= Qtot: row stride of the record
+};
+hipError_t launch_region_sums(const RegionArgs &a, hipStream_t s);
 
 }  // namespace sbe

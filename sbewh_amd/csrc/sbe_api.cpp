@@ -1,7 +1,7 @@
 // Single-instance C ABI of libsbewh_hip.so (include/sbewh_hip.h): context
 // life cycle, pulses, ground state, RHS, observer and the device-resident
 // Dormand-Prince controller.
-#include "context.hpp"
+#include "observer.hpp"
 
 using namespace sbe_host;
 
@@ -107,7 +107,7 @@ int sbe_fermi_level(sbe_ctx *ctx, int occupied_bands, int below_gap, double *mu)
         a.nk = c.nk;
         a.n = n;
         a.P = c.P;
-        a.sweeps = c.prm.sweep_count;
+        a.sweeps = c.sweeps_for(0);
         HIPCHK(launch_init_state(c.N, c.lr ? c.lr : sbe_ctx::default_lr(c.N), a, c.stream));
         std::vector<double> ev((size_t)n * c.nk);
         HIPCHK(hipMemcpyAsync(ev.data(), eig, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -155,7 +155,7 @@ void sbe_host::initial_state_dev(sbe_ctx &c, double mu, double *dst, hipStream_t
     a.nk = c.nk;
     a.n = c.n;
     a.P = c.P;
-    a.sweeps = c.prm.sweep_count;
+    a.sweeps = c.sweeps_for(0);
     a.mu = mu;
     a.kT = c.prm.gs_temp;
     HIPCHK(launch_init_state(c.N, c.lr ? c.lr : sbe_ctx::default_lr(c.N), a, s));
@@ -215,11 +215,21 @@ int sbe_observe(sbe_ctx *ctx, const double *state, double j[3])
 long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out, double *j_out, sbe_steplog *log,
                    const sbe_steplog *replay, long *rhs_calls)
 {
+    return integrate_impl(ctx, state, output_count, t_out, j_out, log, replay, rhs_calls, nullptr);
+}
+
+}  // extern "C"
+
+long sbe_host::integrate_impl(sbe_ctx *ctx, double *state, int output_count, double *t_out, double *j_out,
+                              sbe_steplog *log, const sbe_steplog *replay, long *rhs_calls, sbe_observer *obs)
+{
     if (!ctx || !state) return SBE_ERR_ARG;
     long steps = 0;
     int rc = guarded(ctx, [&] {
         sbe_ctx &c = *ctx;
         if (output_count < 2) fail(SBE_ERR_ARG, "output_count must be >= 2");
+        if (obs && obs->count + output_count > obs->capacity)
+            fail(SBE_ERR_ARG, "observer capacity %d cannot hold %d more records", obs->capacity, output_count);
         if (c.jbuf_cap < output_count) {
             if (c.jbuf) HIPCHK(hipFree(c.jbuf));
             c.jbuf = dalloc<double>(3 * (size_t)output_count);
@@ -239,6 +249,7 @@ long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out,
             if (out < output_count) {
                 tv.push_back(time);
                 c.observe_dev(c.x(), c.jbuf + 3 * out, c.stream);
+                if (obs) obs->record_dev(c.x(), time, c.stream);
             }
             out++;
             double start = time, end = time + dt_out;
@@ -298,6 +309,7 @@ long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out,
         if (out < output_count) {
             tv.push_back(time);
             c.observe_dev(c.x(), c.jbuf + 3 * out, c.stream);
+            if (obs) obs->record_dev(c.x(), time, c.stream);
         }
         out++;
         HIPCHK(hipMemcpyAsync(state, c.x(), 16 * c.S, hipMemcpyDeviceToHost, c.stream));
@@ -312,6 +324,8 @@ long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out,
     if (rhs_calls) *rhs_calls = ctx->rhs_calls;
     return rc ? rc : steps;
 }
+
+extern "C" {
 
 int sbe_hk_mesh(sbe_ctx *ctx, const double A[3], const double E[3], double *out)
 {

@@ -197,6 +197,7 @@ struct sbe_scan {
             a.dephase = c.prm.relaxation_F2 > 0;
             a.ucache = c.ucache;
             a.warm = (c.ucache_valid && !(c.prm.flags & SBE_FLAG_COLD_JACOBI)) ? 1 : 0;
+            a.sweeps = c.sweeps_for(a.warm);
         }
         if (!lr) tune_stage(args);
         const int kpb = stage_k_per_block(N, lr);
@@ -305,6 +306,7 @@ struct sbe_scan {
             o.dim = c.dim;
             o.partial = c.partial;
             o.j_out = d_jout + 3 * ((long)ids[j] * oc + ctl[ids[j]].out);
+            o.pstride = 3;
             b0[j] = tot;
             tot += c.nk;
         }

@@ -735,6 +735,98 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     }
 }
 
+// H0(k + A) of the observer's partial phase sum (op 0 of M-wide rows) for the
+// wave's k-points into LDS natural layout (last-axis contraction per entry)
+template <int N, int LR>
+__device__ __forceinline__ void obs_h0_mats(cd *lds, const OccArgs &a, long blk)
+{
+    using G = Geo<N, LR>;
+    const int n = a.n;
+    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
+        const int kk = e / G::NN, idx = e % G::NN;
+        const int r = idx / N, c = idx % N;
+        const long k = blk * G::KPW + kk;
+        cd v = mk(0, 0);
+        if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, a.M, r * n + c);
+        lds[kk * G::NN + sw<N>(r, c)] = v;
+    }
+}
+
+// Hamiltonian and Wannier occupations per k-point (ObserverCPU::operator(),
+// code/src/ObserverCPU.cpp:211-245): U H0(k + A) U^dagger = diag(e) by the
+// wavefront Jacobi (from the identity), occupations (U rho U^dagger)_bb,
+// bands in ascending energy (ties by index), smeared over energy with
+// weights exp(-((e_s - e_q)/smear)^2) summed over q in ascending order.
+template <int N, int LR>
+__global__ __launch_bounds__(64) void occupation_kernel(OccArgs a)
+{
+    using G = Geo<N, LR>;
+    constexpr int C = G::C;
+    __shared__ cd S0[G::KPW * G::NN];
+    __shared__ cd S1[G::KPW * G::NN];
+    __shared__ double Dg[G::KPW * N];
+    __shared__ cd Ja[G::KPW * N], Jb[G::KPW * N];
+    __shared__ int Fa[G::KPW * N];
+    const long blk = blockIdx.x;
+    const Lane L = lane_id<N, LR>(a.nk, blk);
+    const int n = a.n;
+    cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
+    obs_h0_mats<N, LR>(S1, a, blk);
+    load_mats<N, LR>(S0, a.x, a.P, 8, n, a.nk, blk);
+    wsync();
+    cd A[C], U[C];
+    load_xrow<N, LR>(s1, L, A);
+    wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
+    wsync();
+    store_xrow<N, LR>(s1, L, U);
+    wsync();
+    cd T[C];
+    lds_product<N, LR, 0, 0>(s1, s0, L, T);  // T = U rho
+    // (U rho U^dagger)_ii = sum_j T_ij conj(U_ij) over the row's lanes
+    double o = 0;
+#pragma unroll
+    for (int c = 0; c < C; c++) o += T[c].x * U[c].x + T[c].y * U[c].y;
+    o = seg_sum<LR>(o);
+    const double ei = Dg[L.kk * N + L.i];
+    int rank = 0;
+    for (int j = 0; j < n; j++) {
+        const double ej = Dg[L.kk * N + j];
+        rank += (ej < ei) || (ej == ei && j < L.i);
+    }
+    wsync();
+    if (L.h == 0 && L.i < n) Ja[L.kk * N + rank] = mk(ei, o);
+    wsync();
+    if (L.h == 0 && L.i < n && L.valid) {
+        const int sidx = L.i;  // the band this lane reports (ascending energy)
+        const cd es = Ja[L.kk * N + sidx];
+        double v = es.y;
+        if (a.smear > 0) {
+            double ws = 0, acc = 0;
+            for (int q = 0; q < n; q++) {
+                const cd eq = Ja[L.kk * N + q];
+                const double dE = (es.x - eq.x) / a.smear;
+                const double w = exp(-dE * dE);
+                ws += w;
+                acc += eq.y * w;
+            }
+            v = acc / ws;
+        }
+        double *row = a.X + L.k * (long)a.Q;
+        row[3 + sidx] = v;
+        row[3 + n + sidx] = s0[sw<N>(sidx, sidx)].x;
+    }
+}
+
+template <int N, int LR>
+static hipError_t launch_occ_NL(const OccArgs &a, hipStream_t s)
+{
+    using G = Geo<N, LR>;
+    long blocks = (a.nk + G::KPW - 1) / G::KPW;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((occupation_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int N, int LR>
 static hipError_t launch_rhs_NL(const StageArgs &a, hipStream_t s)
 {
@@ -874,6 +966,24 @@ hipError_t launch_rhs_stage_batched(int N, int LR, const StageArgs *args, const 
     case 804: return launch_rhs_batched_NL<8, 4>(args, block0, m, blocks, s);
     case 1602: return launch_rhs_batched_NL<16, 2>(args, block0, m, blocks, s);
     case 1604: return launch_rhs_batched_NL<16, 4>(args, block0, m, blocks, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_occupations(int N, int LR, const OccArgs &a, hipStream_t s)
+{
+    if (a.Q < 3 + 2 * a.n) return hipErrorInvalidValue;
+    switch (N * 100 + LR) {
+    case 201: return launch_occ_NL<2, 1>(a, s);
+    case 202: return launch_occ_NL<2, 2>(a, s);
+    case 401: return launch_occ_NL<4, 1>(a, s);
+    case 402: return launch_occ_NL<4, 2>(a, s);
+    case 404: return launch_occ_NL<4, 4>(a, s);
+    case 801: return launch_occ_NL<8, 1>(a, s);
+    case 802: return launch_occ_NL<8, 2>(a, s);
+    case 804: return launch_occ_NL<8, 4>(a, s);
+    case 1602: return launch_occ_NL<16, 2>(a, s);
+    case 1604: return launch_occ_NL<16, 4>(a, s);
     }
     return hipErrorInvalidValue;
 }

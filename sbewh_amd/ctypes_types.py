@@ -43,6 +43,29 @@ class SbeInstance(C.Structure):
                 ("relaxation_F2", C.c_double)]
 
 
+class SbeRegion(C.Structure):
+    _fields_ = [("k", C.c_double * 3), ("fwhm", C.c_double), ("moving_frame", C.c_int)]
+
+
+def region_array(regions) -> "C.Array":
+    """regions: sequence of (k[3], fwhm, moving_frame) or objects with those
+    attributes; fwhm = inf is the whole zone"""
+    arr = (SbeRegion * max(1, len(regions)))()
+    for i, r in enumerate(regions):
+        k, fwhm, mv = (r.k, r.fwhm, r.moving_frame) if hasattr(r, "fwhm") else r
+        for d in range(3):
+            arr[i].k[d] = float(k[d])
+        arr[i].fwhm = float(fwhm)
+        arr[i].moving_frame = int(bool(mv))
+    return arr
+
+
+class SbeObservations(C.Structure):
+    _fields_ = [(name, C.POINTER(C.c_double)) for name in
+                ("t", "E", "A", "weight_sum", "j", "occ_ham", "occ_wan", "occ_ham_min", "occ_ham_max", "wan")] + \
+               [("density_index", C.POINTER(C.c_int))]
+
+
 class SbeSteplog(C.Structure):
     _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("t", C.POINTER(C.c_double)),
                 ("dt", C.POINTER(C.c_double)), ("err", C.POINTER(C.c_double)),

@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import load, check, SbeError
-from .ctypes_types import Inputs, StepLog, dptr, pulse_array
+from .ctypes_types import Inputs, SbeObservations, StepLog, dptr, pulse_array, region_array
 from .model import Config
 
 
@@ -111,16 +111,21 @@ class SBEContext:
         return j
 
     def integrate(self, state: np.ndarray, output_count: int, replay: StepLog | None = None,
-                  log_capacity: int = 400000) -> dict:
-        """device-resident integrate_const; `state` is advanced in place."""
+                  log_capacity: int = 400000, observer=None) -> dict:
+        """device-resident integrate_const; `state` is advanced in place;
+        observer: an optional FullObserver recording every output time."""
         assert state.dtype == np.complex128 and state.flags.c_contiguous
         t = np.zeros(output_count)
         j = np.zeros((output_count, 3))
         log = StepLog(log_capacity)
         rhs = C.c_long()
-        steps = load().sbe_integrate(self.h, dptr(state.view(float)), output_count, dptr(t), dptr(j.reshape(-1)),
-                                     C.byref(log.c), C.byref(replay.c) if replay is not None else None,
-                                     C.byref(rhs))
+        rp = C.byref(replay.c) if replay is not None else None
+        if observer is not None:
+            steps = load().sbe_integrate_observed(self.h, dptr(state.view(float)), output_count, observer.h, dptr(t),
+                                                  dptr(j.reshape(-1)), C.byref(log.c), rp, C.byref(rhs))
+        else:
+            steps = load().sbe_integrate(self.h, dptr(state.view(float)), output_count, dptr(t),
+                                         dptr(j.reshape(-1)), C.byref(log.c), rp, C.byref(rhs))
         check(steps, self.h)
         return {"t": t, "j": j, "steps": int(steps), "rhs_calls": int(rhs.value), "log": log}
 
@@ -206,10 +211,94 @@ class Observer:
                 "expectationValues/j": np.array(self.j)}
 
 
-def integrate_const(ctx: SBEContext, state: np.ndarray, output_count: int, observer: Observer | None = None,
+class Region:
+    """One k-space region sample (ObserverRegion_t, code/src/ObserverBase.h:33-37):
+    centre k (fractional), fwhm (fractional; inf = whole zone), moving frame."""
+
+    def __init__(self, k, fwhm=float("inf"), moving_frame=True, name="expectationValues"):
+        self.k = tuple(float(x) for x in k)
+        self.fwhm = float(fwhm)
+        self.moving_frame = bool(moving_frame)
+        self.name = name
+
+
+WHOLE_ZONE = Region((0.0, 0.0, 0.0))
+
+
+class FullObserver:
+    """ObserverCPU with k-space regions, occupations and density matrices
+    (code/src/ObserverCPU.cpp:137-308) on the GPU.  ``record(state, t)`` is
+    ObserverCPU::operator(); ``collectExpectationValues()`` returns the
+    records like ExpectationValues_t (code/src/ExpectationValues.h:9-28):
+    t, E, A, occupationHamMin / Max [t][n], and per region sample
+    kPointWeightSum [t], j [t][3], occupationHam / occupationWan [t][n],
+    wan [tDens][n][n] with tDensIndices."""
+
+    def __init__(self, ctx: SBEContext, regions=(WHOLE_ZONE,), smearing_width: float = -1.0,
+                 density_stride: int = 0, capacity: int = 4096):
+        self.ctx = ctx
+        self.regions = list(regions)
+        h = C.c_void_p()
+        arr = region_array(self.regions)
+        check(load().sbe_observer_create(ctx.h, arr, len(self.regions), smearing_width, density_stride, capacity,
+                                         C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().sbe_observer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        check(load().sbe_observer_reset(self.h), self.ctx.h)
+
+    def __call__(self, state: np.ndarray, t: float) -> None:
+        self.record(state, t)
+
+    def record(self, state: np.ndarray, t: float) -> None:
+        state = np.ascontiguousarray(state, complex)
+        check(load().sbe_observer_record(self.h, dptr(state.view(float)), t), self.ctx.h)
+
+    def count(self):
+        dc = C.c_int()
+        c = load().sbe_observer_count(self.h, C.byref(dc))
+        check(c, self.ctx.h)
+        return c, dc.value
+
+    def collectExpectationValues(self) -> dict:
+        cnt, dcnt = self.count()
+        R, n = len(self.regions), self.ctx.n
+        out = {"t": np.zeros(cnt), "E": np.zeros((cnt, 3)), "A": np.zeros((cnt, 3)),
+               "kPointWeightSum": np.zeros((cnt, R)), "j": np.zeros((cnt, R, 3)),
+               "occupationHam": np.zeros((cnt, R, n)), "occupationWan": np.zeros((cnt, R, n)),
+               "occupationHamMin": np.zeros((cnt, n)), "occupationHamMax": np.zeros((cnt, n)),
+               "wan": np.zeros((dcnt, R, n, n), complex), "tDensIndices": np.zeros(dcnt, np.int32)}
+        ob = SbeObservations()
+        for key, name in (("t", "t"), ("E", "E"), ("A", "A"), ("kPointWeightSum", "weight_sum"), ("j", "j"),
+                          ("occupationHam", "occ_ham"), ("occupationWan", "occ_wan"),
+                          ("occupationHamMin", "occ_ham_min"), ("occupationHamMax", "occ_ham_max")):
+            setattr(ob, name, dptr(out[key].reshape(-1)))
+        ob.wan = dptr(out["wan"].view(float).reshape(-1)) if dcnt else None
+        ob.density_index = out["tDensIndices"].ctypes.data_as(C.POINTER(C.c_int)) if dcnt else None
+        check(load().sbe_observer_results(self.h, C.byref(ob)), self.ctx.h)
+        return out
+
+
+def integrate_const(ctx: SBEContext, state: np.ndarray, output_count: int, observer=None,
                     replay: StepLog | None = None) -> int:
     """integrate_const(controlled dopri5, propagator, state, t0, t1 + dt/2, dt, observer),
-    device resident; returns the number of accepted steps."""
+    device resident; returns the number of accepted steps.  observer: an
+    Observer (current only) or a FullObserver (recorded on the device at every
+    output time)."""
+    if isinstance(observer, FullObserver):
+        res = ctx.integrate(state, output_count, replay=replay, observer=observer)
+        return res["steps"]
     res = ctx.integrate(state, output_count, replay=replay)
     if observer is not None:
         observer.reset()
@@ -220,4 +309,5 @@ def integrate_const(ctx: SBEContext, state: np.ndarray, output_count: int, obser
     return res["steps"]
 
 
-__all__ = ["SBEContext", "Propagator", "Observer", "integrate_const", "jacobi_batch", "SbeError"]
+__all__ = ["SBEContext", "Propagator", "Observer", "FullObserver", "Region", "WHOLE_ZONE", "integrate_const",
+           "jacobi_batch", "SbeError"]

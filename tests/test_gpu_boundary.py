@@ -14,6 +14,7 @@ from sbewh_amd.ctypes_types import Inputs, dptr
 from sbewh_amd.model import Pulse
 from sbewh_amd.propagator import SBEContext
 from oracle.oracle_py import Oracle
+from sbewh_amd import hiprt as hip_mem
 from tests import cases, cpp_driver
 from tests.test_oracle_dopri5 import uniform_log
 
@@ -83,7 +84,9 @@ def test_k_index_map_matches_fourier_block_order():
 
 
 def test_rhs_device_gpu_layout_on_device_buffers():
-    torch = pytest.importorskip("torch")
+    """sbe_rhs_device on caller-owned device buffers in the GPU layout (the
+    CuPropagator GPUstate: interleaved doubles), enqueued on a caller stream,
+    equals sbe_rhs on host buffers bit for bit"""
     cfg = chain_short(True)
     cg = SBEContext(cfg)
     n, nk = cg.n, cg.num_k
@@ -92,12 +95,17 @@ def test_rhs_device_gpu_layout_on_device_buffers():
     x[0] = 0.2
     t = 0.4 * cg.time_window[0]
     dx = cg.rhs(x, t)
-    d_in = torch.from_numpy(x.view(np.float64).copy()).cuda()   # GPUstate: interleaved doubles
-    d_out = torch.zeros_like(d_in)
-    torch.cuda.synchronize()
-    cg.rhs_device(d_in.data_ptr(), d_out.data_ptr(), t, torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
-    assert np.array_equal(d_out.cpu().numpy(), dx.view(np.float64))
+    flat = x.view(np.float64)
+    d_in = hip_mem.DeviceBuffer.from_array(flat)
+    d_out = hip_mem.DeviceBuffer.from_array(np.zeros_like(flat))
+    s = hip_mem.Stream()
+    cg.rhs_device(d_in.ptr, d_out.ptr, t, s.handle)
+    s.synchronize()
+    assert np.array_equal(d_out.download(np.float64, flat.size), dx.view(np.float64))
+    s.destroy()
+    d_in.free()
+    d_out.free()
+    cg.close()
 
 
 def test_integrate_gpu_layout_equals_cpu_layout():
@@ -138,5 +146,7 @@ def test_cpp_functor_plain_dopri5_against_oracle(gpu_layout):
     st_cpu = to_cpu_layout(st, nk, n) if gpu_layout else st
     rho_g, rho_o = st_cpu[8:].reshape(nk, o.P)[:, :n * n], o.rho(x).reshape(nk, n * n)
     assert np.abs(rho_g - rho_o).max() < 1e-10 * np.abs(rho_o).max()
-    assert np.abs(st_cpu[:3] - x[:3]).max() < 1e-10 * np.abs(x[:3]).max()
+    # A returns to ~0 at the end of the pulse: scale by its peak along the pulse
+    A_peak = max(np.abs(o.A(t)).max() for t in np.linspace(t0, t1, 201))
+    assert np.abs(st_cpu[:3] - x[:3]).max() < 1e-10 * A_peak
     assert np.abs(j - r["j"]).max() < 1e-10 * np.abs(r["j"]).max()
