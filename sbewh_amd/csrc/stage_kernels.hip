@@ -297,6 +297,57 @@ __device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
     for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))];
 }
 
+// ---- 16 x 16 complex products on the FP64 matrix cores (16-band layouts) --
+// v_mfma_f64_16x16x4f64: A (16 x 4) lane l holds A[l % 16][l / 16], B (4 x 16)
+// lane l holds B[l / 16][l % 16], the 16 x 16 accumulator lane l holds rows
+// 4 j + l / 16, j = 0..3, of column l % 16.  A complex product is four real
+// products of 16 x 16 x 16, each four MFMA steps along the inner index.
+typedef double d4 __attribute__((ext_vector_type(4)));
+struct CAcc {
+    d4 r, i;
+};
+
+__device__ __forceinline__ void cacc_zero(CAcc &c)
+{
+    c.r = d4{0, 0, 0, 0};
+    c.i = d4{0, 0, 0, 0};
+}
+
+// acc += SX * X Y for one k-point's matrices in LDS natural layout
+// (XOP 1: X[i][l] := conj(Xs[l][i]); YOP 1: Y[l][j] := conj(Ys[j][l]));
+// every operand is read before the first matrix instruction
+template <int XOP, int YOP, int SX = 1>
+__device__ __forceinline__ void mfma_cprod(const cd *X, const cd *Y, CAcc &acc)
+{
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    cd a[4], b[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+        const int k = 4 * kc + q;
+        a[kc] = (XOP == 0) ? X[sw<16>(r, k)] : cconj(X[sw<16>(k, r)]);
+        b[kc] = (YOP == 0) ? Y[sw<16>(k, r)] : cconj(Y[sw<16>(r, k)]);
+        if (SX < 0) a[kc] = mk(-a[kc].x, -a[kc].y);
+    }
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+        acc.r = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].x, b[kc].x, acc.r, 0, 0, 0);
+        acc.i = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].x, b[kc].y, acc.i, 0, 0, 0);
+        acc.r = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[kc].y, b[kc].y, acc.r, 0, 0, 0);
+        acc.i = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].y, b[kc].x, acc.i, 0, 0, 0);
+    }
+}
+
+// accumulator element j of this lane: row and column
+__device__ __forceinline__ int cacc_row(int j) { return 4 * j + ((threadIdx.x & 63) >> 4); }
+__device__ __forceinline__ int cacc_col() { return threadIdx.x & 15; }
+
+// the accumulator into LDS natural layout (one k-point)
+__device__ __forceinline__ void cacc_store(cd *lds, const CAcc &c)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) lds[sw<16>(cacc_row(j), cacc_col())] = mk(c.r[j], c.i[j]);
+}
+
 __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s)
 {
     const double t = y / (fabs(x) + sqrt(fma(x, x, y * y)));
@@ -522,12 +573,189 @@ __device__ __forceinline__ double stage_err(const StageArgs &a, const cd *lds, l
     return v;
 }
 
+// special slots of the stage: d A_t/dt = -E_t (t < dim), and the stage input
+// itself where it is stored (block 0)
+__device__ __forceinline__ void stage_specials(const StageArgs &a, const long blk)
+{
+    if (blk == 0 && threadIdx.x < 8) {
+        const int t = threadIdx.x;
+        double v = t < a.dim ? -a.E[t] : 0.0;
+        a.dxdt[2 * t] = v;
+        a.dxdt[2 * t + 1] = 0.0;
+        if (a.xout) {
+            double re = a.in.x[2 * t], im = a.in.x[2 * t + 1];
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+                if (j < a.in.nt) {
+                    re += a.in.c[j] * a.in.k[j][2 * t];
+                    im += a.in.c[j] * a.in.k[j][2 * t + 1];
+                }
+            a.xout[2 * t] = re;
+            a.xout[2 * t + 1] = im;
+        }
+    }
+}
+
+// The stage for 16-band layouts: the Jacobi in the XOR-wrapped register
+// layout, every matrix product on the matrix cores (mfma_cprod), one k-point
+// of the block at a time with all 64 lanes; products hand over through LDS
+// in natural layout.  Same operations as the generic body below.
+template <int LR>
+__device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long blk)
+{
+    constexpr int N = 16;
+    using G = Geo<N, LR>;
+    constexpr int C = G::C, KPW = G::KPW;
+    __shared__ cd S0[KPW * G::NN];
+    __shared__ cd S1[KPW * G::NN];
+    __shared__ cd S2[KPW * G::NN];
+    __shared__ double Dg[KPW * N];
+    __shared__ cd Ja[KPW * N], Jb[KPW * N];
+    __shared__ int Fa[KPW * N];
+
+    const Lane L = lane_id<N, LR>(a.nk, blk);
+    const int n = a.n, nn = n * n;
+    cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
+    stage_specials(a, blk);
+
+    if (a.dephase) {
+        cd A[C], U[C];
+        if (a.warm) {
+            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
+            load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
+            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
+            wsync();
+            load_xrow<N, LR>(s0, L, U);
+#pragma unroll
+            for (int c = 0; c < C; c++)
+                if (L.i >= n) U[c] = mk((L.h * C + c) == 0 ? 1.0 : 0.0, 0);
+            CAcc T[KPW];
+#pragma unroll
+            for (int kk = 0; kk < KPW; kk++) {
+                cacc_zero(T[kk]);
+                mfma_cprod<0, 0>(S0 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // U' H0
+            }
+            wsync();
+#pragma unroll
+            for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, T[kk]);
+            store_xrow<N, LR>(s1, L, U);
+            wsync();
+#pragma unroll
+            for (int kk = 0; kk < KPW; kk++) {
+                cacc_zero(T[kk]);
+                mfma_cprod<0, 1>(S0 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // (U' H0) U'^dagger
+            }
+            wsync();
+#pragma unroll
+            for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, T[kk]);
+            wsync();
+            load_xrow<N, LR>(s0, L, A);
+            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
+        } else {
+            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
+            wsync();
+            load_xrow<N, LR>(s1, L, A);
+            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
+        }
+        store_xrow<N, LR>(s1, L, U);  // U -> S1
+        wsync();
+        store_ucache<N, LR>(a, S1, blk);
+    }
+    load_rho<N, LR>(S0, a, blk);  // rho -> S0 (S1 still holds U)
+    wsync();
+    if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
+    // the dephasing first, on the side buffer S2 (rho stays in S0), then the
+    // coherent part, accumulated onto its last product: the result lives in
+    // registers only from there to the store
+    CAcc D[KPW];
+#pragma unroll
+    for (int kk = 0; kk < KPW; kk++) cacc_zero(D[kk]);
+    if (a.dephase) {
+        // U^dagger (D o (U rho U^dagger)) U
+        CAcc T[KPW];
+#pragma unroll
+        for (int kk = 0; kk < KPW; kk++) {
+            cacc_zero(T[kk]);
+            mfma_cprod<0, 0>(S1 + kk * G::NN, S0 + kk * G::NN, T[kk]);  // T = U rho
+            cacc_store(S2 + kk * G::NN, T[kk]);
+        }
+        wsync();
+#pragma unroll
+        for (int kk = 0; kk < KPW; kk++) {
+            cacc_zero(T[kk]);
+            mfma_cprod<0, 1>(S2 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // rhoH = T U^dagger
+            const int b = cacc_col();
+            const double eb = Dg[kk * N + b];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int ar = cacc_row(j);
+                double gam;
+                if (a.w < 0) {
+                    gam = (ar != b) ? 1.0 : 0.0;
+                } else {
+                    const double dE = (Dg[kk * N + ar] - eb) / a.w;
+                    gam = 1 - exp(-dE * dE);
+                }
+                const double f = -gam * a.F2;
+                T[kk].r[j] *= f;
+                T[kk].i[j] *= f;
+            }
+        }
+        wsync();
+#pragma unroll
+        for (int kk = 0; kk < KPW; kk++) cacc_store(S2 + kk * G::NN, T[kk]);
+        wsync();
+#pragma unroll
+        for (int kk = 0; kk < KPW; kk++) {
+            cacc_zero(T[kk]);
+            mfma_cprod<0, 0>(S2 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // S = rhoH' U
+        }
+        wsync();
+#pragma unroll
+        for (int kk = 0; kk < KPW; kk++) cacc_store(S2 + kk * G::NN, T[kk]);
+        wsync();
+#pragma unroll
+        for (int kk = 0; kk < KPW; kk++) mfma_cprod<1, 0>(S1 + kk * G::NN, S2 + kk * G::NN, D[kk]);  // U^dagger S
+        wsync();
+    }
+    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
+    wsync();
+    // coherent part -i (H_E rho - rho H_E): D + (-i) X = D + (X.i, -X.r), so
+    // it is accumulated as i (rho H_E - H_E rho) with the roles of the real and
+    // imaginary accumulators exchanged
+#pragma unroll
+    for (int kk = 0; kk < KPW; kk++) {
+        CAcc t;
+        t.r = D[kk].i;
+        t.i = -D[kk].r;
+        mfma_cprod<0, 0, -1>(S1 + kk * G::NN, S0 + kk * G::NN, t);  // - H_E rho
+        mfma_cprod<0, 0>(S0 + kk * G::NN, S1 + kk * G::NN, t);      // + rho H_E
+        // t = i D + (rho H_E - H_E rho) = i (D - i (H_E rho - rho H_E))
+        D[kk].r = -t.i;
+        D[kk].i = t.r;
+    }
+    // d rho/dt (with zero pads) through LDS for coalescing
+    wsync();
+#pragma unroll
+    for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, D[kk]);
+    wsync();
+    store_state<N, LR>(a.dxdt, S0, n, a.P, a.nk, blk);
+    if (a.err_partial) {
+        const double v = seg_max<64>(stage_err<N, LR>(a, S0, blk));
+        if (threadIdx.x == 0) a.err_partial[blk] = v;
+    }
+}
+
 // the stage for the KPW k-points of logical block blk of one instance
 template <int N, int LR>
 __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
 {
     using G = Geo<N, LR>;
     constexpr int C = G::C;
+    if constexpr (N == 16) {
+        rhs_stage_body16<LR>(a, blk);
+        return;
+    } else {
     __shared__ cd S0[G::KPW * G::NN];
     __shared__ cd S1[G::KPW * G::NN];
     __shared__ double Dg[G::KPW * N];
@@ -650,10 +878,18 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
         const double v = seg_max<64>(stage_err<N, LR>(a, S0, blk));
         if (threadIdx.x == 0) a.err_partial[blk] = v;
     }
+    }
 }
 
+// minimum waves per SIMD the register allocation must allow: the 16-band
+// kernels fit three (158 VGPRs) when the allocator is told so, instead of
+// two with part of the matrix-core operands in AGPRs
+template <int N>
+constexpr int stage_min_waves() { return N == 16 ? 3 : 1; }
+
 template <int N, int LR>
-__global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N>())))
+void rhs_stage_kernel(StageArgs a)
 {
     rhs_stage_body<N, LR>(a, xcd_block(blockIdx.x, gridDim.x));
 }
@@ -661,7 +897,8 @@ __global__ __launch_bounds__(64) void rhs_stage_kernel(StageArgs a)
 // Several instances in one launch: entry j owns the logical blocks
 // [block0[j], block0[j+1]); its arguments are read once into registers.
 template <int N, int LR>
-__global__ __launch_bounds__(64) void rhs_stage_batched_kernel(const StageArgs *args, const long *block0, int m)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N>())))
+void rhs_stage_batched_kernel(const StageArgs *args, const long *block0, int m)
 {
     const long gb = xcd_block(blockIdx.x, gridDim.x);
     const int j = find_entry(block0, m, gb);
