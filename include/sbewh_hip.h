@@ -103,6 +103,18 @@ int sbe_rhs_device(sbe_ctx *ctx, const double *d_state, double *d_dsdt, double t
  * region, code/src/ObserverCPU.h:43, code/src/ObserverCPU.cpp:149-210,293-308). */
 int sbe_observe(sbe_ctx *ctx, const double *state, double j[3]);
 
+/* No counterpart in the reference: sbe_observe for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * d_state: device state in the context's layout; d_j: device buffer of 3
+ * doubles that receives j.  Enqueued on `stream` (NULL: the context's
+ * stream); returns without synchronising.  Same kernels and summation order
+ * as sbe_observe.  The per-k partial currents go to a file-scope device array
+ * of 3 * 262144 doubles (6 MiB, held on each GPU the library runs on until the
+ * process ends), so grids of at most 262144 k-points are accepted (SBE_ERR_ARG
+ * beyond).  Every call on one GPU uses that GPU's array: calls that overlap
+ * on one GPU race. */
+int sbe_observe_device_noalloc(sbe_ctx *ctx, const double *d_state, double *d_j, void *stream);
+
 /* Whole propagation, device resident:
  * integrate_const(make_controlled<runge_kutta_dopri5>(eps_abs, eps_rel), rhs, state,
  *                 t0, t1 + dt_out/2, dt_out, observer)

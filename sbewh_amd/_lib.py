@@ -22,7 +22,7 @@ EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_se
            "sbe_scan_destroy", "sbe_scan_last_error", "sbe_scan_run", "sbe_scan_state_size", "sbe_scan_final_state",
            "sbe_scan_tuning", "sbe_observer_create", "sbe_observer_destroy", "sbe_observer_reset",
            "sbe_observer_record", "sbe_observer_record_device", "sbe_observer_count", "sbe_observer_results",
-           "sbe_integrate_observed")
+           "sbe_integrate_observed", "sbe_observe_device_noalloc")
 # every symbol include/sbewh_surrogate.h declares
 SURROGATE_EXPORTS = ("sbe_mlp_create", "sbe_mlp_destroy", "sbe_mlp_last_error", "sbe_mlp_param_count",
                      "sbe_mlp_get_params", "sbe_mlp_set_params", "sbe_mlp_forward", "sbe_mlp_gradient",
@@ -86,6 +86,7 @@ def load():
     L.sbe_scan_state_size.argtypes = [vp, i]
     L.sbe_scan_final_state.argtypes = [vp, i, dp]
     L.sbe_scan_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), dp]
+    L.sbe_observe_device_noalloc.argtypes = [vp, vp, vp, vp]
     L.sbe_observer_create.argtypes = [vp, P(SbeRegion), i, d, i, i, P(vp)]
     L.sbe_observer_destroy.argtypes = [vp]
     L.sbe_observer_reset.argtypes = [vp]
