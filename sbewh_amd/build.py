@@ -23,7 +23,8 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{SRC}"]
 SOURCES = ["phase_kernels.hip", "stage_kernels.hip", "rk_kernels.hip", "observer_kernels.hip", "sbe_api.cpp",
-           "observer_api.cpp", "scan_api.cpp", "spectrum.cpp", "mlp_kernels.hip", "mlp_api.cpp", "noalloc.hip"]
+           "observer_api.cpp", "scan_api.cpp", "spectrum.cpp", "mlp_kernels.hip", "mlp_api.cpp", "noalloc.hip",
+           "w90.cpp"]
 
 
 def _local_includes(path: str, seen=None) -> list:

@@ -4,8 +4,9 @@ A model is what the reference reads from a Wannier90 ``_tb.dat`` file after
 degeneracy division, symmetrisation and dimensional projection
 (code/src/io/TightBindingParameter.cpp:287-329): a list of lattice cells R with
 Hamiltonian blocks H(R) and position blocks r_d(R), plus the lattice vectors,
-all in atomic units.  ``TBModel`` holds exactly that; ``tbfile`` writes and
-reads it in the Wannier90 ``_tb.dat`` format.
+all in atomic units.  ``TBModel`` holds exactly that; ``sbewh_amd.w90`` reads
+it from Wannier90 ``_tb.dat`` / ``_wsvec.dat`` files and writes it in the
+``_tb.dat`` layout.
 
 The configurations mirror the workloads the benchmark is quoted on:
 
