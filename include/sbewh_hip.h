@@ -222,6 +222,29 @@ int sbe_hhg_spectrum(const double *t, const double *j, int nt, int ncomp, double
                      double omega_max, double fadeout_rel, int power, double *omega_out, double *S_out,
                      int capacity, int *count);
 
+/* The non-FFT branch of calcSpectrum (scripts/util/common.py:187-201):
+ * sbe_direct_omega is directOmega (scripts/util/hhg.py:27-38): the grid
+ * omega_min, omega_min + d, ... below omega_max, with omega_max lowered to the
+ * Nyquist limit pi/dt and d raised to pi/(t_end - t_0) where needed (count
+ * receives the length; nothing written beyond capacity, SBE_ERR_ARG then);
+ * sbe_direct_spectrum is directSpectrum (hhg.py:40-42) of one real signal:
+ * S(omega) = |omega^power sum_i exp(i omega t_i) j_i|^2; sbe_hhg_spectrum_direct
+ * is sbe_hhg_spectrum with the direct sum in place of the FFT (same j
+ * normalisation and component sum, no fadeout). */
+int sbe_direct_omega(const double *t, int nt, double omega_min, double omega_max, double d_omega, double *omega_out,
+                     int capacity, int *count);
+int sbe_direct_spectrum(const double *t, int nt, const double *omega, int nw, const double *j, int power,
+                        double *S_out);
+int sbe_hhg_spectrum_direct(const double *t, const double *j, int nt, int ncomp, double d_omega, double omega_min,
+                            double omega_max, int power, double *omega_out, double *S_out, int capacity, int *count);
+
+/* smearSpectrum (scripts/util/hhg.py:45-49): Gaussian filter of full width at
+ * half maximum smear_width (in omega units) over the uniform grid omega, zero
+ * outside, kernel truncated at 4 standard deviations and normalised (the
+ * scipy.ndimage.gaussian_filter1d call); smear_width < 0 copies S.  out may
+ * alias S. */
+int sbe_smear_spectrum(const double *omega, const double *S, int count, double smear_width, double *out);
+
 /* Per-harmonic yield Y_h = sum of S over omega in [h w0 - width w0, h w0 + width w0),
  * h = 1..hmax (extractProfiles, scripts/ellipticityDependency.py:77-83);
  * omega ascending. */

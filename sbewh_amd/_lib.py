@@ -22,7 +22,8 @@ EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_se
            "sbe_scan_destroy", "sbe_scan_last_error", "sbe_scan_run", "sbe_scan_state_size", "sbe_scan_final_state",
            "sbe_scan_tuning", "sbe_observer_create", "sbe_observer_destroy", "sbe_observer_reset",
            "sbe_observer_record", "sbe_observer_record_device", "sbe_observer_count", "sbe_observer_results",
-           "sbe_integrate_observed", "sbe_observe_device_noalloc")
+           "sbe_integrate_observed", "sbe_observe_device_noalloc", "sbe_direct_omega", "sbe_direct_spectrum",
+           "sbe_hhg_spectrum_direct", "sbe_smear_spectrum")
 # every symbol include/sbewh_surrogate.h declares
 SURROGATE_EXPORTS = ("sbe_mlp_create", "sbe_mlp_destroy", "sbe_mlp_last_error", "sbe_mlp_param_count",
                      "sbe_mlp_get_params", "sbe_mlp_set_params", "sbe_mlp_forward", "sbe_mlp_gradient",
@@ -76,6 +77,10 @@ def load():
     L.sbe_stage_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), dp]
     L.sbe_hhg_spectrum.argtypes = [dp, dp, i, i, d, d, d, d, i, dp, dp, i, P(C.c_int)]
     L.sbe_harmonic_yields.argtypes = [dp, dp, i, d, d, i, dp]
+    L.sbe_direct_omega.argtypes = [dp, i, d, d, d, dp, i, P(C.c_int)]
+    L.sbe_direct_spectrum.argtypes = [dp, i, dp, i, dp, i, dp]
+    L.sbe_hhg_spectrum_direct.argtypes = [dp, dp, i, i, d, d, d, i, dp, dp, i, P(C.c_int)]
+    L.sbe_smear_spectrum.argtypes = [dp, dp, i, d, dp]
     L.sbe_scan_create.argtypes = [P(SbeModel), P(SbeGrid), P(SbeParams), P(SbeInstance), i, P(SbeSubpulse), i,
                                   P(vp)]
     L.sbe_scan_destroy.argtypes = [vp]

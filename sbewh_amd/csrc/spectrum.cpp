@@ -9,6 +9,10 @@
 //   showHHG        scripts/showExpectationValues.py:36-58 (j / max|j|, per-direction
 //                                              spectra summed, directions below 1e-10 max dropped)
 //   extractProfiles scripts/ellipticityDependency.py:77-83 (sum over [h w0 - w, h w0 + w))
+//   directOmega    scripts/util/hhg.py:27-38  (omega grid limited by the sampling theorem)
+//   directSpectrum scripts/util/hhg.py:40-42  (S = |omega^power sum_t exp(i omega t) j(t)|^2)
+//   smearSpectrum  scripts/util/hhg.py:45-49  (Gaussian filter of FWHM smear_width, zero outside,
+//                                              scipy.ndimage.gaussian_filter1d with truncate 4)
 // The discrete Fourier transform of arbitrary length uses Bluestein's chirp-z
 // algorithm over an iterative radix-2 FFT (no external FFT library).
 #include <algorithm>
@@ -144,6 +148,98 @@ int sbe_hhg_spectrum(const double *t, const double *j, int nt, int ncomp, double
         if (S_out) S_out[q] = total[k];
     }
     *count = (int)sel.size();
+    return SBE_OK;
+}
+
+int sbe_direct_omega(const double *t, int nt, double omega_min, double omega_max, double d_omega, double *omega_out,
+                     int capacity, int *count)
+{
+    if (!t || nt < 2 || !(d_omega > 0) || !count) return SBE_ERR_ARG;
+    const double nyq_max = M_PI / (t[1] - t[0]), nyq_d = M_PI / (t[nt - 1] - t[0]);
+    if (!(nyq_max > 0) || !(nyq_d > 0)) return SBE_ERR_ARG;
+    if (nyq_max < omega_max) omega_max = nyq_max;
+    if (nyq_d > d_omega) d_omega = nyq_d;
+    // numpy.arange(omega_min, omega_max, d_omega)
+    const double len = std::ceil((omega_max - omega_min) / d_omega);
+    const long n = len > 0 ? (long)len : 0;
+    *count = (int)n;
+    if (n > capacity) return SBE_ERR_ARG;
+    if (omega_out)
+        for (long i = 0; i < n; i++) omega_out[i] = omega_min + (double)i * d_omega;
+    return SBE_OK;
+}
+
+int sbe_direct_spectrum(const double *t, int nt, const double *omega, int nw, const double *j, int power,
+                        double *S_out)
+{
+    if (!t || nt < 1 || !omega || nw < 0 || !j || !S_out) return SBE_ERR_ARG;
+    for (int q = 0; q < nw; q++) {
+        cplx acc = 0;
+        for (int i = 0; i < nt; i++) acc += std::exp(cplx(0, omega[q] * t[i])) * j[i];
+        S_out[q] = std::norm(std::pow(omega[q], power) * acc);
+    }
+    return SBE_OK;
+}
+
+int sbe_hhg_spectrum_direct(const double *t, const double *j, int nt, int ncomp, double d_omega, double omega_min,
+                            double omega_max, int power, double *omega_out, double *S_out, int capacity, int *count)
+{
+    if (!t || !j || nt < 2 || ncomp < 1 || ncomp > 3 || !count) return SBE_ERR_ARG;
+    int nw = 0;
+    int rc = sbe_direct_omega(t, nt, omega_min, omega_max, d_omega, nullptr, 0, &nw);
+    if (rc && nw == 0) return rc;
+    *count = nw;
+    if (nw > capacity) return SBE_ERR_ARG;
+    std::vector<double> om(nw);
+    int c2 = 0;
+    sbe_direct_omega(t, nt, omega_min, omega_max, d_omega, om.data(), nw, &c2);
+    double gmax = 0;
+    std::vector<double> cmax(ncomp, 0.0);
+    for (int c = 0; c < ncomp; c++)
+        for (int i = 0; i < nt; i++) cmax[c] = std::max(cmax[c], std::fabs(j[(size_t)i * ncomp + c]));
+    for (double v : cmax) gmax = std::max(gmax, v);
+    std::vector<double> total(nw, 0.0), S(nw), r(nt);
+    if (gmax > 0)
+        for (int c = 0; c < ncomp; c++) {
+            if (cmax[c] < 1e-10 * gmax) continue;
+            for (int i = 0; i < nt; i++) r[i] = j[(size_t)i * ncomp + c] / gmax;
+            sbe_direct_spectrum(t, nt, om.data(), nw, r.data(), power, S.data());
+            for (int q = 0; q < nw; q++) total[q] += S[q];
+        }
+    for (int q = 0; q < nw; q++) {
+        if (omega_out) omega_out[q] = om[q];
+        if (S_out) S_out[q] = total[q];
+    }
+    return SBE_OK;
+}
+
+int sbe_smear_spectrum(const double *omega, const double *S, int count, double smear_width, double *out)
+{
+    if (!omega || !S || !out || count < 0) return SBE_ERR_ARG;
+    if (smear_width < 0 || count < 2) {
+        std::memmove(out, S, sizeof(double) * (size_t)count);
+        return SBE_OK;
+    }
+    const double sigma = smear_width / (omega[1] - omega[0]) / (2 * std::sqrt(2 * std::log(2.0)));
+    if (!(sigma > 0)) return SBE_ERR_ARG;
+    const long rad = (long)(4.0 * sigma + 0.5);
+    std::vector<double> w(2 * rad + 1);
+    double sum = 0;
+    for (long k = -rad; k <= rad; k++) {
+        w[k + rad] = std::exp(-0.5 * (double)k * (double)k / (sigma * sigma));
+        sum += w[k + rad];
+    }
+    for (auto &x : w) x /= sum;
+    std::vector<double> res(count, 0.0);
+    for (long i = 0; i < count; i++) {
+        double acc = 0;
+        for (long k = -rad; k <= rad; k++) {
+            const long p = i + k;
+            if (p >= 0 && p < count) acc += w[k + rad] * S[p];
+        }
+        res[i] = acc;
+    }
+    std::memcpy(out, res.data(), sizeof(double) * (size_t)count);
     return SBE_OK;
 }
 

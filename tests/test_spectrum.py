@@ -59,3 +59,59 @@ def test_library_matches_reference_restatement(ncomp, nt):
     y = harmonic_yields(om, S, w0, 0.5, 40)
     ok = y_r > 0
     assert np.abs(np.log10(y[ok]) - np.log10(y_r[ok])).max() < 1e-8
+
+
+def test_direct_spectrum_restatement_is_the_dft_at_fft_frequencies():
+    """pin of the oracle's directSpectrum: at omega_k = 2 pi k / (N dt) the
+    direct sum over a zero-padded real signal is N times the FFT spectrum
+    (|sum_n e^{+i w_k t_n} j_n| = |DFT_k| for real j, any time origin)"""
+    t = np.linspace(-50.0, 49.0, 100)
+    x = np.random.default_rng(3).normal(size=100)
+    dw = 2 * np.pi / (t[1] - t[0]) / 160                 # N = 160
+    om, S_fft = ref.fft_spectrum(t, dw, x.copy(), power=1, rel_fadeout=0.0)
+    xx = x.copy()
+    xx[-1] = 0                                            # fadeout(rel=0) zeroes the last sample
+    S_dir = ref.direct_spectrum(t, om, xx, power=1)
+    assert np.allclose(S_dir, 160 * S_fft, rtol=1e-10, atol=1e-12 * S_dir.max())
+    # the grid: Nyquist-limited maximum, window-limited spacing
+    g = ref.direct_omega(t, 0.0, 10.0, 1e-6)
+    assert np.isclose(g[1] - g[0], np.pi / (t[-1] - t[0])) and g[-1] < np.pi / (t[1] - t[0])
+
+
+def test_smear_restatement_limits():
+    """pin of the oracle's smearSpectrum: it conserves the sum away from the
+    edges, leaves a constant unchanged in the interior, and turns a delta
+    into a Gaussian of the requested FWHM"""
+    om = np.arange(400) * 0.01
+    S = np.zeros(400)
+    S[200] = 1.0
+    out = ref.smear_spectrum(om, S, 0.2)
+    assert abs(out.sum() - 1.0) < 1e-12
+    half = out.max() / 2
+    above = om[out >= half]
+    fwhm = above[-1] - above[0]
+    assert abs(fwhm - 0.2) <= 0.011                       # one grid step
+    flat = ref.smear_spectrum(om, np.ones(400), 0.2)
+    assert np.allclose(flat[50:350], 1.0, rtol=1e-12)
+    assert np.array_equal(ref.smear_spectrum(om, S, -1.0), S)
+
+
+@pytest.mark.parametrize("ncomp,nt", [(1, 801), (3, 641)])
+def test_library_direct_and_smear_match_restatement(ncomp, nt):
+    from sbewh_amd.spectrum import direct_omega, direct_spectrum, hhg_spectrum_direct, smear_spectrum
+    t, j, w0 = _signal(nt, seed=10 + ncomp, ncomp=ncomp)
+    dw, wmax = units.from_eV(0.02), units.from_eV(15.0)
+    assert np.array_equal(direct_omega(t, 0.0, wmax, dw), ref.direct_omega(t, 0.0, wmax, dw))
+    om = ref.direct_omega(t, 0.0, wmax, dw)
+    S1 = direct_spectrum(t, om, j[:, 0])
+    S1r = ref.direct_spectrum(t, om, j[:, 0])
+    assert np.abs(S1 - S1r).max() <= 1e-11 * S1r.max()
+    om_l, S_l = hhg_spectrum_direct(t, j, 0.02, 0.0, 15.0)
+    om_r, S_r = ref.hhg_spectrum_direct(t, j, dw, 0.0, wmax)
+    assert np.array_equal(om_l, om_r)
+    above = S_r > 1e-10 * S_r.max()
+    assert np.abs(np.log10(S_l[above]) - np.log10(S_r[above])).max() < 1e-8
+    for width in (0.5 * w0, 0.05 * w0, -1.0):
+        sm_l = smear_spectrum(om_r, S_r, width)
+        sm_r = ref.smear_spectrum(om_r, S_r, width)
+        assert np.abs(sm_l - sm_r).max() <= 1e-12 * sm_r.max()
