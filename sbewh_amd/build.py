@@ -89,9 +89,26 @@ def build(force: bool = False, verbose: bool = True) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{r.stderr}")
         os.replace(LIB + ".tmp", LIB)
+    build_cli(force or not _newer(CLI, [LIB] + [os.path.join(CLI_SRC, f) for f in os.listdir(CLI_SRC)]))
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+CLI_SRC = os.path.join(SRC, "cli")
+CLI = os.path.join(LIBDIR, "sbewh_run")
+
+
+def build_cli(needed: bool = True) -> str:
+    """the command line front end (host C++ over the C ABI), next to the library"""
+    if needed:
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CLI_SRC}", os.path.join(CLI_SRC, "run_sbe.cpp"),
+               f"-L{LIBDIR}", "-lsbewh_hip", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"sbewh_run build failed:\n{r.stderr}")
+        os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":
