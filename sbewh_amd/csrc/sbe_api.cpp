@@ -236,6 +236,9 @@ long sbe_host::integrate_impl(sbe_ctx *ctx, double *state, int output_count, dou
             c.jbuf_cap = output_count;
         }
         c.rhs_calls = 0;
+        // every integration starts its Jacobi chain from the identity, so that
+        // its results do not depend on what the context computed before
+        c.ucache_valid = false;
         if (log) log->count = 0;
         HIPCHK(hipMemcpyAsync(c.x(), state, 16 * c.S, hipMemcpyHostToDevice, c.stream));
         const double t0 = c.t_start(), t1 = c.t_end();
