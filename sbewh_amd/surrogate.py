@@ -131,17 +131,26 @@ def log_yields(t: np.ndarray, j: np.ndarray, lambda_nm: float) -> np.ndarray:
     return np.log10(np.maximum(y, LOG_FLOOR))
 
 
-def solve(params: np.ndarray, grid: int = 15, base=None, output_count: int = 1024):
-    """log10 yields [n, 40] of the solver (one batched scan), seconds, steps"""
+def solve(params: np.ndarray, grid: int = 15, base=None, output_count: int = 1024, chunk: int = 0):
+    """log10 yields [n, 40] of the solver (batched scans of `chunk` pulses,
+    all in one when 0, with a progress line per scan), seconds, steps"""
     from .scan import Scan
     base = base or model.config("square2d")
     cfgs = [config_for(p, base, grid, output_count) for p in params]
+    chunk = chunk or len(cfgs)
     t0 = time.perf_counter()
-    sc = Scan(cfgs)
-    r = sc.run()
-    sc.close()
-    ly = np.array([log_yields(r["t"][i], r["j"][i], cfgs[i].pulses[0].lambda_nm) for i in range(len(cfgs))])
-    return ly, time.perf_counter() - t0, r["steps"]
+    ly, steps = [], []
+    for a in range(0, len(cfgs), chunk):
+        part = cfgs[a:a + chunk]
+        sc = Scan(part)
+        r = sc.run()
+        sc.close()
+        ly += [log_yields(r["t"][i], r["j"][i], part[i].pulses[0].lambda_nm) for i in range(len(part))]
+        steps += list(r["steps"])
+        if chunk < len(cfgs):
+            print(f"[solve] {grid}^2 k: {a + len(part)} of {len(cfgs)} spectra, {time.perf_counter() - t0:.1f} s",
+                  flush=True)
+    return np.array(ly), time.perf_counter() - t0, np.array(steps)
 
 
 # ---- the surrogate -------------------------------------------------------------
@@ -217,7 +226,15 @@ def _main():
     g.add_argument("--count", type=int, default=1024)
     g.add_argument("--seed", type=int, default=5000)
     g.add_argument("--grid", type=int, default=15)
+    g.add_argument("--chunk", type=int, default=0, help="pulses per batched scan (0: all in one)")
     g.add_argument("--out", required=True)
+    k = sub.add_parser("kconv", help="k-grid convergence of the log10 yields on held-out pulses")
+    k.add_argument("--count", type=int, default=32)
+    k.add_argument("--seed", type=int, default=9000)
+    k.add_argument("--grids", default="15,30,45,60,150")
+    k.add_argument("--chunk", type=int, default=8)
+    k.add_argument("--ensemble", default=os.path.join(DATA_DIR, "surrogate_ensemble.npz"))
+    k.add_argument("--out", required=True)
     t = sub.add_parser("train", help="train the ensemble on generated data")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--test-fraction", type=float, default=0.125)
@@ -226,10 +243,30 @@ def _main():
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--out", required=True)
     a = ap.parse_args()
-    if a.cmd == "gen":
+    if a.cmd == "kconv":
+        params = sample_params(a.count, a.seed)
+        grids = [int(g) for g in a.grids.split(",")]
+        res = {}
+        for g in grids:
+            ly, secs, steps = solve(params, g, chunk=a.chunk)
+            res[g] = ly
+            print(f"[kconv] {g}^2 k: {secs:.1f} s", flush=True)
+        ref = res[max(grids)]
+        s = Surrogate.load(a.ensemble)
+        mean, _ = s.predict(params)
+        report = {"count": a.count, "reference_grid": max(grids),
+                  "median_abs_log10_vs_reference": {str(g): float(np.median(np.abs(res[g] - ref))) for g in grids},
+                  "p90_abs_log10_vs_reference": {str(g): float(np.quantile(np.abs(res[g] - ref), 0.9))
+                                                 for g in grids},
+                  "surrogate_median_abs_log10_vs_reference": float(np.median(np.abs(mean - ref))),
+                  "surrogate_median_rel_log10_vs_reference": float(np.median(relative_log_error(mean, ref)))}
+        np.savez_compressed(a.out, params=params, grids=np.array(grids),
+                            **{f"log_yields_{g}": res[g] for g in grids})
+        print(json.dumps(report), flush=True)
+    elif a.cmd == "gen":
         params = sample_params(a.count, a.seed)
         print(f"[gen] {a.count} spectra on {a.grid}^2 k-points, seed {a.seed}", flush=True)
-        ly, secs, steps = solve(params, a.grid)
+        ly, secs, steps = solve(params, a.grid, chunk=a.chunk)
         np.savez_compressed(a.out, params=params, log_yields=ly, seconds=secs, steps=steps, grid=a.grid,
                             seed=a.seed)
         print(json.dumps({"count": a.count, "seconds": secs, "spectra_per_hour": a.count / secs * 3600,
