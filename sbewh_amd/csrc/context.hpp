@@ -510,7 +510,12 @@ struct sbe_ctx {
                 if (q == hipErrorNotReady) continue;
                 HIPCHK(q);
                 b = *p;
-                if (b == ERR_PENDING) throw HipFail{hipErrorUnknown, "error norm not delivered"};
+                if (b == ERR_PENDING) {
+                    // a fused launch that ended without its last block leaves
+                    // the ticket counter non-zero: clear it for the next attempt
+                    if (err_ticket) (void)hipMemsetAsync(err_ticket, 0, sizeof(unsigned), s);
+                    throw HipFail{hipErrorUnknown, "error norm not delivered"};
+                }
                 break;
             }
         }

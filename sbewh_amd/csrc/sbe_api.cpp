@@ -239,6 +239,8 @@ long sbe_host::integrate_impl(sbe_ctx *ctx, double *state, int output_count, dou
         // every integration starts its Jacobi chain from the identity, so that
         // its results do not depend on what the context computed before
         c.ucache_valid = false;
+        // the fused attempt's ticket counter starts from zero
+        HIPCHK(hipMemsetAsync(c.err_ticket, 0, sizeof(unsigned), c.stream));
         if (log) log->count = 0;
         HIPCHK(hipMemcpyAsync(c.x(), state, 16 * c.S, hipMemcpyHostToDevice, c.stream));
         const double t0 = c.t_start(), t1 = c.t_end();
@@ -400,6 +402,7 @@ int sbe_bench_steps(sbe_ctx *ctx, const double *state, double t, double dt, int 
     return guarded(ctx, [&] {
         sbe_ctx &c = *ctx;
         HIPCHK(hipMemcpyAsync(c.x(), state, 16 * c.S, hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemsetAsync(c.err_ticket, 0, sizeof(unsigned), c.stream));
         c.rhs_dev(c.x(), c.st[1], t, c.stream);
         // the controller of integrate_adaptive without output truncation
         auto run = [&](int count) {

@@ -4,11 +4,13 @@ Tolerances: FP64 throughout.  Single kernels: <= 1e-12 relative.  Whole
 propagations replaying the oracle's accepted step sequence: j(t) and the final
 density matrices <= 1e-10 relative (the target stated for this path).
 """
+import dataclasses
 import sys
 
 import numpy as np
 import pytest
 
+from sbewh_amd import model
 from sbewh_amd.propagator import SBEContext, jacobi_batch
 from oracle.oracle_py import Oracle
 from tests import cases
@@ -226,6 +228,57 @@ def test_one_block_attempt_kernel(monkeypatch):
             check_error_norms(rr, ro)
         g.close()
     (r0, s0), (r1, s1) = res["512"], res["0"]
+    say(f"  steps {r0['steps']} / {r1['steps']} (oracle {ro['steps']})")
+    assert r0["steps"] == r1["steps"]
+    assert np.array_equal(r0["j"], r1["j"])
+    assert np.array_equal(s0, s1)
+
+
+@pytest.fixture(scope="module")
+def cubic_segment():
+    """16-band 9^3 model over a short pulse window (2 um, 8 MV/cm, +-0.05 fs),
+    rtol 1e-7: the oracle's run at a converged sweep count (8 cyclic sweeps
+    from the identity on every call)"""
+    base = cases.small_cubic()
+    pul = [model.Pulse("sin2", 0.8, 2000.0, 0.0, 0.05)]
+    cfg_o = dataclasses.replace(base, sweeps=8, eps_rel=1e-7, pulses=pul, output_count=8)
+    o = Oracle(cfg_o)
+    x0 = o.initial_state(o.fermi_level(cfg_o.occupied_bands))
+    xo = x0.copy()
+    say("[gpu] cubic 9^3 n=16 segment: oracle propagation")
+    ro = o.integrate(xo, cfg_o.output_count)
+    o.close()
+    return cfg_o, x0, xo, ro
+
+
+@pytest.mark.parametrize("lr", [2, 4])
+def test_sixteen_band_attempt_kernel_layouts(monkeypatch, cubic_segment, lr):
+    """16 bands, layout LR lanes per row: the attempt in one launch (fused,
+    its last block reducing the error norm) gives the same bits as one launch
+    per stage under the GPU's own step control, and the fused path replays the
+    oracle's step sequence with j(t) and rho <= 1e-10 and matching error norms
+    (GPU: at most 5 warm-started sweeps per RHS)."""
+    cfg_o, x0, xo, ro = cubic_segment
+    cfg = dataclasses.replace(cfg_o, sweeps=5)
+    say(f"[gpu] cubic 9^3 n=16 LR={lr}: one launch per attempt vs one per stage")
+    monkeypatch.setenv("SBEWH_STAGE_LR", str(lr))
+    res = {}
+    for fuse in ("100000", "0"):
+        monkeypatch.setenv("SBEWH_FUSE_BELOW", fuse)
+        g = SBEContext(cfg)
+        s = x0.copy()
+        r = g.integrate(s, cfg.output_count)
+        assert g.stage_tuning()["lanes_per_row"] == lr
+        res[fuse] = (r, s)
+        if fuse == "100000":
+            sr = x0.copy()
+            rr = g.integrate(sr, cfg.output_count, replay=ro["log"])
+            say(f"  fused replay: j rel {rel(rr['j'], ro['j']):.2e}, rho rel {rel(sr, xo):.2e}")
+            assert rel(rr["j"], ro["j"]) <= 1e-10
+            assert rel(sr, xo) <= 1e-10
+            check_error_norms(rr, ro)
+        g.close()
+    (r0, s0), (r1, s1) = res["100000"], res["0"]
     say(f"  steps {r0['steps']} / {r1['steps']} (oracle {ro['steps']})")
     assert r0["steps"] == r1["steps"]
     assert np.array_equal(r0["j"], r1["j"])
