@@ -362,17 +362,18 @@ __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s
     }
 }
 
-// conj(a) * b
-__device__ __forceinline__ cd cmul_cj(cd a, cd b) { return cd{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
-
 // One round r of the parallel Jacobi sweep: rotations on the pairs (p, q = p ^ r).
-// The rotation J of a pair acts on rows, row p <- J_pp row p + J_pq row q and
-// row q <- J_qq row q + J_qp row p, and as J^dagger on columns,
-// column j <- conj(J_jj) column j + conj(J_j,j^r) column j ^ r.  The owner of
-// element (p, q) stores (J_jj, J_j,j^r) under index j for j = p and j = q
-// (the identity when the pair does not rotate), so every lane reads its
-// row's and its columns' coefficients without selects or divergent branches;
-// an identity coefficient pair reproduces its operand exactly.
+// With a_pq = m w (m = |a_pq|, |w| = 1) the rotation of a pair is
+//   J = [[c, s w], [-s conj(w), c]],  (c, s) = (cos t, sin t), t = atan2(2m, a_pp - a_qq) / 2,
+// which zeroes (J A J^dagger)_pq; its diagonal is real.  It acts on rows,
+// row p <- c row p + s w row q and row q <- c row q - s conj(w) row p, and as
+// J^dagger on columns, column j <- J_jj column j + conj(J_j,j^r) column j ^ r.
+// Lane q takes a_pq from lane p, so both lanes of a pair compute the same
+// rotation from the same operands.  Each stores its row's coefficients
+// (J_jj, J_j,j^r) under index j (the identity when the pair does not
+// rotate), so every lane reads its row's and its columns' coefficients
+// without selects or divergent branches; an identity coefficient pair
+// reproduces its operand exactly.
 template <int N, int LR, int R>
 __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double thr, double *Dg, cd *Ja, cd *Jb,
                                           int *Fa, int &anyrot)
@@ -383,47 +384,43 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
     constexpr int mrow = R * LR + rh;   // lane mask of the row partner (row i ^ R)
     const int base = L.kk * N;
     // (a) rotation of the pair (p, q = p ^ R).  Lanes h == rh of rows p and q
-    // hold elements (p, q) and (q, p) = conj(a_pq) in register rc; lane p
-    // computes the phase half-angle of a_pq, lane q the rotation half-angle,
-    // and they swap results (lane ^ R LR).  Both store their row's coefficients.
+    // hold elements (p, q) and (q, p) = conj(a_pq) in register rc.
     {
         constexpr int mpair = R * LR;
         const bool isp = L.i < (L.i ^ R);
         const cd e = A[rc];
-        // |a_pq| from lane p for both lanes (lane q's conj(a_pq) comes from
-        // other roundings), so that the pair's two lanes agree on act
-        // (every exchange runs in all lanes: a DPP read from a lane that is
-        // switched off returns 0)
-        const double mine = sqrt(fma(e.x, e.x, e.y * e.y));
+        const double thr2 = thr * thr;
         // a round in which no pair of the wavefront rotates leaves A and U
-        // unchanged: skip it before computing any rotation
-        if (!__any(L.h == rh && isp && L.valid && mine > thr)) return;
+        // unchanged: skip it before computing any rotation (every exchange
+        // runs in all lanes: a DPP read from a lane that is switched off
+        // returns 0)
+        if (!__any(L.h == rh && isp && L.valid && fma(e.x, e.x, e.y * e.y) > thr2)) return;
         anyrot = 1;
-        const double other = xchg<mpair>(mk(mine, 0)).x;
-        const double mod = isp ? mine : other;
+        const cd o = xchg<mpair>(e);
+        const cd apq = isp ? e : o;
         const int pr = isp ? L.i : (L.i ^ R);
-        const double app = Dg[base + pr], aqq = Dg[base + (pr ^ R)];
-        double ch, sh;
-        cs_half(isp ? -e.x : 2 * mod, isp ? e.y : app - aqq, ch, sh);
-        const cd o = xchg<mpair>(mk(ch, sh));
-        const double c1 = isp ? ch : o.x, s1 = isp ? sh : o.y;
-        const double c2 = isp ? o.x : ch, s2 = isp ? o.y : sh;
         if (L.h == rh) {
-            const int act = L.valid && (mod > thr);
-            cd ja = mk(1, 0), jb = mk(0, 0);
+            const double m2 = fma(apq.x, apq.x, apq.y * apq.y);
+            const int act = L.valid && (m2 > thr2);
+            double ja = 1, jbx = 0, jby = 0;
             if (act) {
-                // row p: (J_pp, J_pq);  row q: (J_qq, J_qp) = (conj(J_pp), -conj(J_pq))
-                ja = mk(-s1 * s2, isp ? -c1 * s2 : c1 * s2);
-                jb = mk(isp ? -c1 * c2 : c1 * c2, -s1 * c2);
+                const double ri = rsqrt(m2);
+                double c, sn;
+                cs_half(2 * (m2 * ri), Dg[base + pr] - Dg[base + (pr ^ R)], c, sn);
+                const double swx = sn * (apq.x * ri), swy = sn * (apq.y * ri);
+                ja = c;
+                jbx = isp ? swx : -swx;
+                jby = swy;
             }
-            Ja[base + L.i] = ja;
-            Jb[base + L.i] = jb;
+            Ja[base + L.i] = mk(ja, 0);
+            Jb[base + L.i] = mk(jbx, jby);
             if (isp) Fa[base + L.i] = act;
         }
     }
     wsync();
     const int arow = Fa[base + min(L.i, L.i ^ R)];
-    const cd ca = Ja[base + L.i], cb = Jb[base + L.i];
+    const double ca = Ja[base + L.i].x;
+    const cd cb = Jb[base + L.i];
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const int c2 = c ^ rc;
@@ -432,11 +429,11 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         cd a0 = A[c], a1 = A[c2];
         const cd b0 = xchg<mrow>(A[c2]), b1 = xchg<mrow>(A[c]);
         const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
-        a0 = cfma(cb, b0, cmul(ca, a0));
-        U[c] = cfma(cb, v0, cmul(ca, U[c]));
+        a0 = cfma(cb, b0, cscale(a0, ca));
+        U[c] = cfma(cb, v0, cscale(U[c], ca));
         if (c2 != c) {
-            a1 = cfma(cb, b1, cmul(ca, a1));
-            U[c2] = cfma(cb, v1, cmul(ca, U[c2]));
+            a1 = cfma(cb, b1, cscale(a1, ca));
+            U[c2] = cfma(cb, v1, cscale(U[c2], ca));
         } else {
             a1 = a0;
         }
@@ -449,24 +446,24 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
             d0 = xchg<rh>(a1);
             d1 = xchg<rh>(a0);
         }
-        const int g0 = L.h * C + c, g1 = L.h * C + c2;
-        const int j0 = L.i ^ g0;
-        const cd ja = Ja[base + j0], jb = Jb[base + j0];
-        // the rotated pivot (i, i ^ R) is set to zero
-        cd r0 = cfma_cj(jb, d0, cmul_cj(ja, a0));
-        if (g0 == R && arow) r0 = mk(0, 0);
+        const int j0 = L.i ^ (L.h * C + c);
+        const double ja = Ja[base + j0].x;
+        const cd jb = Jb[base + j0];
+        cd r0 = cfma_cj(jb, d0, cscale(a0, ja));
+        // the rotated pivot (i, i ^ R) is set to zero: it sits in register rc of lanes h == rh
+        if (c == rc && L.h == rh && arow) r0 = mk(0, 0);
         A[c] = r0;
         if (c2 != c) {
             cd r1;
             if constexpr (rh == 0) {
-                // column j1 = j0 ^ R is j0's partner: (J_j1j1, J_j1j0) = (conj(J_j0j0), -conj(J_j0j1))
-                r1 = cfma(mk(-jb.x, -jb.y), d1, cmul(ja, a1));
+                // column j1 = j0 ^ R is j0's partner: (J_j1j1, J_j1j0) = (J_j0j0, -conj(J_j0j1))
+                r1 = cfma(mk(-jb.x, -jb.y), d1, cscale(a1, ja));
             } else {
-                // column j1 = i ^ g1 belongs to another pair (its partner is in lane h ^ rh)
-                const int j1 = L.i ^ g1;
-                r1 = cfma_cj(Jb[base + j1], d1, cmul_cj(Ja[base + j1], a1));
+                // column j1 = i ^ (h C + c2) belongs to another pair (its partner is in lane h ^ rh)
+                const int j1 = L.i ^ (L.h * C + c2);
+                r1 = cfma_cj(Jb[base + j1], d1, cscale(a1, Ja[base + j1].x));
             }
-            if (g1 == R && arow) r1 = mk(0, 0);
+            if (c2 == rc && L.h == rh && arow) r1 = mk(0, 0);
             A[c2] = r1;
         }
     }
