@@ -312,7 +312,8 @@ def main():
         for other in ("cubic3d", "square2d", "chain1d"):
             if other != args.config:
                 r = measure(other, max(args.steps, 50), args.warmup, cpu_seconds=None)
-                out[other] = {k: r["line"][k] for k in ("value", "unit", "ms_per_step", "roofline", "config")}
+                out[other] = {k: r["line"][k] for k in ("value", "unit", "ms_per_step", "roofline", "config",
+                                                         "breakdown_ms_per_rhs", "stage_tuning")}
     print(json.dumps(out), flush=True)
 
 

@@ -297,6 +297,67 @@ __device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
     for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))];
 }
 
+// The Jacobi's eigenvector registers.  With jac_ut, lane (i, h) register c
+// holds U[i ^ (h C + c)][i] (the xrow layout of U^T): a rotation of rows
+// (j, j ^ r) of U then combines register c with register c ^ (r mod C) of the
+// same lane (r < C) or of lane h ^ (r / C) in the row's own lane group, and it
+// uses the coefficients the column update of A reads anyway, instead of an
+// exchange with the partner row's lanes.  For 16 bands at 4 lanes per row
+// that trade would replace LDS permutes by more vector-ALU moves, so that
+// layout keeps U in rows.
+template <int N, int LR>
+constexpr bool jac_ut() { return !(N == 16 && LR == 4); }
+
+// U registers (either layout) <-> LDS natural layout
+template <int N, int LR>
+__device__ __forceinline__ void store_u(cd *lds, const Lane &L, const cd *v)
+{
+    using G = Geo<N, LR>;
+    if constexpr (jac_ut<N, LR>()) {
+#pragma unroll
+        for (int c = 0; c < G::C; c++) lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)] = v[c];
+    } else {
+        store_xrow<N, LR>(lds, L, v);
+    }
+}
+
+template <int N, int LR>
+__device__ __forceinline__ void load_u(const cd *lds, const Lane &L, cd *v)
+{
+    using G = Geo<N, LR>;
+    if constexpr (jac_ut<N, LR>()) {
+#pragma unroll
+        for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)];
+    } else {
+        load_xrow<N, LR>(lds, L, v);
+    }
+}
+
+// padded rows and columns (index >= n) of U are the identity
+template <int N, int LR>
+__device__ __forceinline__ void u_pad_identity(const Lane &L, int n, cd *v)
+{
+    using G = Geo<N, LR>;
+#pragma unroll
+    for (int c = 0; c < G::C; c++) {
+        const int g = L.h * G::C + c;
+        if (L.i >= n || (L.i ^ g) >= n) v[c] = mk(g == 0 ? 1.0 : 0.0, 0);
+    }
+}
+
+// U registers of the Jacobi -> xrow layout (rows), through the k-point's
+// N x N LDS region lds (free on entry, rewritten)
+template <int N, int LR>
+__device__ __forceinline__ void u_rows(cd *lds, const Lane &L, cd *v)
+{
+    if constexpr (jac_ut<N, LR>()) {
+        store_u<N, LR>(lds, L, v);
+        wsync();
+        load_xrow<N, LR>(lds, L, v);
+        wsync();
+    }
+}
+
 // ---- 16 x 16 complex products on the FP64 matrix cores (16-band layouts) --
 // v_mfma_f64_16x16x4f64: A (16 x 4) lane l holds A[l % 16][l / 16], B (4 x 16)
 // lane l holds B[l / 16][l % 16], the 16 x 16 accumulator lane l holds rows
@@ -428,12 +489,14 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         // row update of the register pair (c, c2); the partner row holds them in (c2, c)
         cd a0 = A[c], a1 = A[c2];
         const cd b0 = xchg<mrow>(A[c2]), b1 = xchg<mrow>(A[c]);
-        const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
         a0 = cfma(cb, b0, cscale(a0, ca));
-        U[c] = cfma(cb, v0, cscale(U[c], ca));
+        if constexpr (!jac_ut<N, LR>()) {
+            const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
+            U[c] = cfma(cb, v0, cscale(U[c], ca));
+            if (c2 != c) U[c2] = cfma(cb, v1, cscale(U[c2], ca));
+        }
         if (c2 != c) {
             a1 = cfma(cb, b1, cscale(a1, ca));
-            U[c2] = cfma(cb, v1, cscale(U[c2], ca));
         } else {
             a1 = a0;
         }
@@ -449,6 +512,28 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         const int j0 = L.i ^ (L.h * C + c);
         const double ja = Ja[base + j0].x;
         const cd jb = Jb[base + j0];
+        if constexpr (jac_ut<N, LR>()) {
+            // U^T registers: row j0 of U (register c) pairs with row j0 ^ R (register c2 of lane h ^ rh)
+            const cd u0 = U[c], u1 = U[c2];
+            cd e0, e1;
+            if constexpr (rh == 0) {
+                e0 = u1;
+                e1 = u0;
+            } else {
+                e0 = xchg<rh>(u1);
+                e1 = xchg<rh>(u0);
+            }
+            U[c] = cfma(jb, e0, cscale(u0, ja));
+            if (c2 != c) {
+                if constexpr (rh == 0) {
+                    // row j1 = j0 ^ R: (J_j1j1, J_j1j0) = (J_j0j0, -conj(J_j0j1))
+                    U[c2] = cfma(mk(-jb.x, jb.y), e1, cscale(u1, ja));
+                } else {
+                    const int j1 = L.i ^ (L.h * C + c2);
+                    U[c2] = cfma(Jb[base + j1], e1, cscale(u1, Ja[base + j1].x));
+                }
+            }
+        }
         cd r0 = cfma_cj(jb, d0, cscale(a0, ja));
         // the rotated pivot (i, i ^ R) is set to zero: it sits in register rc of lanes h == rh
         if (c == rc && L.h == rh && arow) r0 = mk(0, 0);
@@ -622,10 +707,8 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
             load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
             phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
             wsync();
-            load_xrow<N, LR>(s0, L, U);
-#pragma unroll
-            for (int c = 0; c < C; c++)
-                if (L.i >= n) U[c] = mk((L.h * C + c) == 0 ? 1.0 : 0.0, 0);
+            load_u<N, LR>(s0, L, U);
+            u_pad_identity<N, LR>(L, n, U);
             CAcc T[KPW];
 #pragma unroll
             for (int kk = 0; kk < KPW; kk++) {
@@ -635,7 +718,7 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
             wsync();
 #pragma unroll
             for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, T[kk]);
-            store_xrow<N, LR>(s1, L, U);
+            store_u<N, LR>(s1, L, U);
             wsync();
 #pragma unroll
             for (int kk = 0; kk < KPW; kk++) {
@@ -654,7 +737,7 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
             load_xrow<N, LR>(s1, L, A);
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
         }
-        store_xrow<N, LR>(s1, L, U);  // U -> S1
+        store_u<N, LR>(s1, L, U);  // U -> S1
         wsync();
         store_ucache<N, LR>(a, S1, blk);
     }
@@ -789,16 +872,14 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
             load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
             phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
             wsync();
-            load_xrow<N, LR>(s0, L, U);
-            // padded rows of U' are the identity
-#pragma unroll
-            for (int c = 0; c < C; c++)
-                if (L.i >= n) U[c] = mk((L.h * C + c) == 0 ? 1.0 : 0.0, 0);
+            load_u<N, LR>(s0, L, U);
+            // padded rows and columns of U' are the identity
+            u_pad_identity<N, LR>(L, n, U);
             cd T[C];
             lds_product<N, LR, 0, 0>(s0, s1, L, T);  // U' H0
             wsync();
             store_xrow<N, LR>(s0, L, T);
-            store_xrow<N, LR>(s1, L, U);
+            store_u<N, LR>(s1, L, U);
             wsync();
             lds_product<N, LR, 0, 1>(s0, s1, L, A);  // (U' H0) U'^dagger
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
@@ -808,7 +889,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
             load_xrow<N, LR>(s1, L, A);
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
         }
-        store_xrow<N, LR>(s1, L, U);  // U -> S1
+        store_u<N, LR>(s1, L, U);  // U -> S1
         wsync();
         // the eigenvectors go to the per-k cache: warm start of the next call,
         // and the copy the dephasing below reloads (each lane reads back only
@@ -879,7 +960,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
 }
 
 // minimum waves per SIMD the register allocation must allow: the 16-band
-// kernels fit three (158 VGPRs) when the allocator is told so, instead of
+// kernels fit three (138 VGPRs at 4 lanes per row) when the allocator is told so, instead of
 // two with part of the matrix-core operands in AGPRs
 template <int N>
 constexpr int stage_min_waves() { return N == 16 ? 3 : 1; }
@@ -924,6 +1005,8 @@ __global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
     cd A[C], U[C];
     load_xrow<N, LR>(s1, L, A);
     wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
+    wsync();
+    u_rows<N, LR>(s1, L, U);
     if (a.eig && L.valid && L.h == 0 && L.i < n) a.eig[L.k * n + L.i] = Dg[L.kk * N + L.i];
     if (a.U && L.valid && L.i < n) {
 #pragma unroll
@@ -1012,6 +1095,7 @@ __global__ __launch_bounds__(64) void occupation_kernel(OccArgs a)
     load_xrow<N, LR>(s1, L, A);
     wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
     wsync();
+    u_rows<N, LR>(s1, L, U);
     store_xrow<N, LR>(s1, L, U);
     wsync();
     cd T[C];
