@@ -55,8 +55,15 @@ def _newer(a: str, deps: list) -> bool:
     return all(os.path.getmtime(d) <= t for d in deps)
 
 
-def _compile(src: str, force: bool) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+# stage_kernels.hip is built as two objects (SBE_STAGE_PART 1 and 2): the
+# one-lane-per-row stage kernels (part 2) with the default machine scheduler,
+# everything else (part 1) with the ILP-oriented one
+UNITS = [(s, p) for s in SOURCES for p in ((1, 2) if s == "stage_kernels.hip" else (0,))]
+
+
+def _compile(unit, force: bool) -> str:
+    src, part = unit
+    obj = os.path.join(OBJ, os.path.basename(src) + (f".part{part}" if part else "") + ".o")
     deps = [os.path.join(SRC, src)] + _local_includes(os.path.join(SRC, src)) + [os.path.abspath(__file__)]
     if not force and _newer(obj, deps):
         return obj
@@ -66,9 +73,12 @@ def _compile(src: str, force: bool) -> str:
     extra = ["-ffp-contract=off"] if src.endswith(".cpp") else []
     # the stage kernels are issue-bound with a third of their cycles waiting;
     # the ILP-oriented machine scheduler hides more of that latency at the same
-    # occupancy (150 x 150 2-D model: 116.8 -> 113.7 us per stage launch;
-    # 48^3 3-D model: 2.99 -> 2.95 ms)
-    if src == "stage_kernels.hip":
+    # occupancy (48^3 3-D model: 2.30 -> 2.14 ms per stage launch); the
+    # one-lane-per-row kernels keep the default scheduler, which fits them in
+    # 162 VGPRs (three waves per SIMD; 150 x 150 2-D model: 108 -> 93 us)
+    if part:
+        extra += [f"-DSBE_STAGE_PART={part}"]
+    if part == 1:
         extra += ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
     cmd = [HIPCC] + COMMON + extra + lang + ["-c", os.path.join(SRC, src), "-o", obj + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -81,8 +91,8 @@ def _compile(src: str, force: bool) -> str:
 def build(force: bool = False, verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(lambda u: _compile(u, force), UNITS))
     if force or not _newer(LIB, objs):
         cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", LIB + ".tmp"] + objs
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -73,6 +73,27 @@ __device__ __forceinline__ int sw(int a, int b)
     }
 }
 
+// LDS layouts of a k-point's N x N matrix (MS entries per k-point):
+// SwLay is the swizzled natural layout above; SlotLay keeps element (a, b) in
+// row a, slot a ^ b, with a row stride of N + 1 entries.  In SlotLay the
+// entries a lane of the one-lane-per-row layout touches in a product,
+// Y[i ^ c'][i ^ c] (slot c' ^ c) and its transposed reads (row i ^ c, slot c),
+// sit at compile-time offsets from one row address, and the odd row stride
+// spreads the rows of a k-point over the LDS banks.
+template <int N>
+struct SwLay {
+    static constexpr int MS = N * N;
+    static constexpr int CHUNK = 64;  // entries per lane whose global loads are in flight together
+    __device__ static int at(int a, int b) { return sw<N>(a, b); }
+};
+template <int N>
+struct SlotLay {
+    static constexpr int RS = N + 1;
+    static constexpr int MS = N * RS;
+    static constexpr int CHUNK = 2;
+    __device__ static int at(int a, int b) { return a * RS + (a ^ b); }
+};
+
 // f(integral_constant<int, c * CH>) for c = 0 .. IT/CH - 1, unrolled at compile
 // time (arrays indexed by the chunk offset stay in registers)
 template <int CH, class F, int... I>
@@ -91,12 +112,12 @@ __device__ __forceinline__ void wsync() { __syncthreads(); }
 
 // coalesced copy of the wave's KPW matrices (n x n at src + k*stride + off,
 // complex) into LDS natural layout [kk][N][N], zero padded.
-template <int N, int LR>
+template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ void load_mats(cd *lds, const double *src, long stride, long off, int n, long nk, long blk)
 {
     using G = Geo<N, LR>;
     constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
-    constexpr int CH = N >= 16 ? 4 : IT;     // entries whose loads are in flight together
+    constexpr int CH = N >= 16 ? 4 : (Lay::CHUNK < IT ? Lay::CHUNK : IT);     // entries whose loads are in flight together
     const double2 *s2 = reinterpret_cast<const double2 *>(src);
     // unconditional loads (masked entries read entry `off` of k-point 0)
     for_chunks<IT, CH>([&](auto i0c) {
@@ -116,7 +137,7 @@ __device__ __forceinline__ void load_mats(cd *lds, const double *src, long strid
         for (int u = 0; u < CH; u++) {
             const int e = threadIdx.x + 64 * (i0 + u);
             const int idx = e % G::NN;
-            lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
+            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
         }
     });
 }
@@ -125,7 +146,7 @@ __device__ __forceinline__ void load_mats(cd *lds, const double *src, long strid
 // natural layout: the last phase-sum contraction
 //   H(k)[a][b] = sum_r tw_L[i_L(k)][r] Y[rest(k)][r][op][a][b]
 // (the k-points of a wave share rest(k), so the Y rows are L1/L2 hits).
-template <int N, int LR>
+template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, long blk)
 {
     using G = Geo<N, LR>;
@@ -140,7 +161,7 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
             const long k = blk * G::KPW + kk;
             cd v = mk(0, 0);
             if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, M, op * nn + r * n + c);
-            lds[kk * G::NN + sw<N>(r, c)] = v;
+            lds[kk * Lay::MS + Lay::at(r, c)] = v;
         }
         return;
     }
@@ -149,7 +170,7 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
     const double2 *Tb = reinterpret_cast<const double2 *>(a.ph.twL);
     // the lane's entries in chunks of CH, all r of a chunk in one unrolled
     // block, so that the loads of different entries are in flight together
-    constexpr int CH = N >= 16 ? 4 : IT;
+    constexpr int CH = N >= 16 ? 4 : (Lay::CHUNK < IT ? Lay::CHUNK : IT);
     for_chunks<IT, CH>([&](auto i0c) {
         constexpr int i0 = decltype(i0c)::value;
         cd acc[CH];
@@ -186,21 +207,21 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
         for (int u = 0; u < CH; u++) {
             const int e = threadIdx.x + 64 * (i0 + u);
             const int idx = e % G::NN;
-            lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? acc[u] : mk(0, 0);
+            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? acc[u] : mk(0, 0);
         }
     });
 }
 
 // stage input rho = x + sum_j c_j k_j (the Dormand-Prince stage combination,
 // left to right as in the reference's scale_sum) into LDS natural layout
-template <int N, int LR>
+template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
 {
     using G = Geo<N, LR>;
     const int n = a.n;
     const StageInput &in = a.in;
     constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
-    constexpr int CH = N >= 16 ? 4 : IT;     // entries whose loads are in flight together
+    constexpr int CH = N >= 16 ? 4 : (Lay::CHUNK < IT ? Lay::CHUNK : IT);     // entries whose loads are in flight together
     for_chunks<IT, CH>([&](auto i0c) {
         constexpr int i0 = decltype(i0c)::value;
         long idx_[CH];
@@ -238,14 +259,14 @@ __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
         for (int u = 0; u < CH; u++) {
             const int e = threadIdx.x + 64 * (i0 + u);
             const int idx = e % G::NN;
-            lds[(e / G::NN) * G::NN + sw<N>(idx / N, idx % N)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
+            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
         }
     });
 }
 
 // the wave's matrices from LDS natural layout into a state vector (reference
 // layout, zero pads), coalesced
-template <int N, int LR>
+template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ void store_state(double *dst, const cd *lds, int n, int P, long nk, long blk)
 {
     using G = Geo<N, LR>;
@@ -256,7 +277,7 @@ __device__ __forceinline__ void store_state(double *dst, const cd *lds, int n, i
         long k = k0 + kk;
         if (k >= nk) break;
         cd v = mk(0, 0);
-        if (p < nn) v = lds[kk * G::NN + sw<N>(p / n, p % n)];
+        if (p < nn) v = lds[kk * Lay::MS + Lay::at(p / n, p % n)];
         reinterpret_cast<double2 *>(dst)[8 + k * P + p] = make_double2(v.x, v.y);
     }
 }
@@ -420,6 +441,21 @@ __device__ __forceinline__ void cs_half(double y, double x, double &c, double &s
     } else {
         c = fabs(ss);
         s = copysign(cc, y);
+    }
+}
+
+// an optimisation barrier: the N accumulators are complete here, and no
+// memory access moves across it (keeps a product's loads per step bounded)
+template <int N>
+__device__ __forceinline__ void pin_rows(cd *v)
+{
+    if constexpr (N == 8) {
+        asm volatile("" : "+v"(v[0].x), "+v"(v[0].y), "+v"(v[1].x), "+v"(v[1].y), "+v"(v[2].x), "+v"(v[2].y),
+                     "+v"(v[3].x), "+v"(v[3].y), "+v"(v[4].x), "+v"(v[4].y), "+v"(v[5].x), "+v"(v[5].y),
+                     "+v"(v[6].x), "+v"(v[6].y), "+v"(v[7].x), "+v"(v[7].y)::"memory");
+    } else {
+#pragma unroll
+        for (int c = 0; c < N; c++) asm volatile("" : "+v"(v[c].x), "+v"(v[c].y)::"memory");
     }
 }
 
@@ -598,7 +634,7 @@ __device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sw
 
 // eigenvector rows from LDS natural layout into the per-k cache [k][n][n],
 // with the lane -> entry mapping of load_mats
-template <int N, int LR>
+template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, long blk)
 {
     using G = Geo<N, LR>;
@@ -609,7 +645,7 @@ __device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, 
         const int r = idx / N, c = idx % N;
         const long k = blk * G::KPW + kk;
         if (r < n && c < n && k < a.nk) {
-            const cd v = lds[kk * G::NN + sw<N>(r, c)];
+            const cd v = lds[kk * Lay::MS + Lay::at(r, c)];
             reinterpret_cast<double2 *>(a.ucache)[k * nn + r * n + c] = make_double2(v.x, v.y);
         }
     }
@@ -619,7 +655,7 @@ __device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, 
 // stage input's x, k1, k3..k6 are read again (HBM that the VALU-bound kernel
 // leaves idle), k7 = d rho/dt comes from LDS natural layout; same lane ->
 // entry mapping as load_rho, two entries' loads in flight at a time.
-template <int N, int LR>
+template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ double stage_err(const StageArgs &a, const cd *lds, long blk)
 {
     using G = Geo<N, LR>;
@@ -638,7 +674,7 @@ __device__ __forceinline__ double stage_err(const StageArgs &a, const cd *lds, l
             double2 kv[6];
 #pragma unroll
             for (int j = 0; j < 5; j++) kv[j] = reinterpret_cast<const double2 *>(a.in.k[j])[i];
-            const cd d = lds[kk * G::NN + sw<N>(r, c)];
+            const cd d = lds[kk * Lay::MS + Lay::at(r, c)];
             kv[5] = make_double2(d.x, d.y);
             v = fmax(v, err_entry(x2[i], kv, a.err_e, a.err_ea, a.err_er, a.err_dt));
         }
@@ -826,6 +862,184 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
     }
 }
 
+// ---- one lane per matrix row (LR = 1, N <= 8): one LDS matrix per k-point --
+// Lane i of a k-point holds row i of a matrix in registers, register c the
+// entry (i, i ^ c) ("rows"), or column i, register c the entry (i ^ c, i)
+// ("columns").  A product takes its left operand from registers and its right
+// operand from the k-point's LDS matrix in SlotLay, so the stage needs one
+// LDS matrix per k-point instead of two (three waves per SIMD fit in LDS).
+template <int N>
+__device__ __forceinline__ void slot_store_rows(cd *s, int i, const cd *v)
+{
+#pragma unroll
+    for (int c = 0; c < N; c++) s[i * SlotLay<N>::RS + c] = v[c];
+}
+template <int N>
+__device__ __forceinline__ void slot_load_rows(const cd *s, int i, cd *v)
+{
+#pragma unroll
+    for (int c = 0; c < N; c++) v[c] = s[i * SlotLay<N>::RS + c];
+}
+template <int N>
+__device__ __forceinline__ void slot_store_cols(cd *s, int i, const cd *v)
+{
+#pragma unroll
+    for (int c = 0; c < N; c++) s[(i ^ c) * SlotLay<N>::RS + c] = v[c];
+}
+template <int N>
+__device__ __forceinline__ void slot_load_cols(const cd *s, int i, cd *v)
+{
+#pragma unroll
+    for (int c = 0; c < N; c++) v[c] = s[(i ^ c) * SlotLay<N>::RS + c];
+}
+
+// out[c] = sum_l X[i][l] Y[l][i ^ c], l = i ^ c'.  X from registers: XOP 0,
+// xr = rows of X; XOP 1, xr = columns of V and X = V^dagger.  Y from LDS
+// (SlotLay): YOP 0, Y = Ys; YOP 1, Y = Ys^dagger.  One register's loads are
+// in flight at a time (the scheduler may not hoist the next ones), which
+// keeps the product's footprint at one row of operands.
+template <int N, int XOP, int YOP>
+__device__ __forceinline__ void reg_product(const cd *xr, const cd *s, int i, cd *out)
+{
+    constexpr int RS = SlotLay<N>::RS;
+#pragma unroll
+    for (int c = 0; c < N; c++) out[c] = mk(0, 0);
+#pragma unroll
+    for (int cp = 0; cp < N; cp++) {
+        const cd x = XOP ? cconj(xr[cp]) : xr[cp];
+        if constexpr (YOP == 0) {
+            // Y[l][j], l = i ^ cp, j = i ^ c: row l, slot cp ^ c
+            const cd *row = s + (i ^ cp) * RS;
+#pragma unroll
+            for (int c = 0; c < N; c++) out[c] = cfma(x, row[cp ^ c], out[c]);
+        } else {
+            // conj(Ys[j][l]): row j = i ^ c, slot c ^ cp
+#pragma unroll
+            for (int c = 0; c < N; c++) out[c] = cfma_jc(x, s[(i ^ c) * RS + (c ^ cp)], out[c]);
+        }
+        pin_rows<N>(out);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long blk)
+{
+    using G = Geo<N, 1>;
+    using Lay = SlotLay<N>;
+    constexpr int C = N, KPW = G::KPW, RS = Lay::RS;
+    __shared__ cd S[KPW * Lay::MS];
+    __shared__ double Dg[KPW * N];
+    __shared__ cd Ja[KPW * N], Jb[KPW * N];
+    __shared__ int Fa[KPW * N];
+
+    const Lane L = lane_id<N, 1>(a.nk, blk);
+    const int n = a.n, nn = n * n;
+    cd *s = S + L.kk * Lay::MS;
+    stage_specials(a, blk);
+
+    cd U[C];   // eigenvectors, columns
+    cd T1[C];  // U rho, rows
+    if (a.dephase) {
+        cd A[C];
+        if (a.warm) {
+            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
+            load_mats<N, 1, Lay>(S, a.ucache, nn, 0, n, a.nk, blk);  // U' -> S
+            wsync();
+            cd T[C];
+            slot_load_rows<N>(s, L.i, T);
+            slot_load_cols<N>(s, L.i, U);
+            // padded rows and columns of U' are the identity
+            u_pad_identity<N, 1>(L, n, T);
+            u_pad_identity<N, 1>(L, n, U);
+            wsync();
+            phase_mats<N, 1, Lay>(S, a, 0, blk);  // H0 -> S
+            wsync();
+            reg_product<N, 0, 0>(T, s, L.i, A);  // U' H0
+            wsync();
+            slot_store_cols<N>(s, L.i, U);  // U' -> S
+            wsync();
+#pragma unroll
+            for (int c = 0; c < C; c++) T[c] = A[c];
+            reg_product<N, 0, 1>(T, s, L.i, A);  // (U' H0) U'^dagger
+            wave_jacobi<N, 1>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
+        } else {
+            phase_mats<N, 1, Lay>(S, a, 0, blk);  // H0 -> S
+            wsync();
+            slot_load_rows<N>(s, L.i, A);
+            wave_jacobi<N, 1>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
+        }
+        wsync();
+        slot_store_cols<N>(s, L.i, U);  // U -> S
+        wsync();
+        store_ucache<N, 1, Lay>(a, S, blk);
+        slot_load_rows<N>(s, L.i, A);  // U rows
+        wsync();
+        load_rho<N, 1, Lay>(S, a, blk);  // rho -> S
+        wsync();
+        reg_product<N, 0, 0>(A, s, L.i, T1);  // U rho
+    } else {
+        load_rho<N, 1, Lay>(S, a, blk);  // rho -> S
+        wsync();
+    }
+    if (a.xout) store_state<N, 1, Lay>(a.xout, S, n, a.P, a.nk, blk);
+    cd dr[C];
+    {
+        // coherent part -i (H_E rho - rho H_E), with H_E rho = (rho H_E)^dagger
+        cd rr[C], T[C];
+        slot_load_rows<N>(s, L.i, rr);
+        wsync();
+        phase_mats<N, 1, Lay>(S, a, 1, blk);  // H_E -> S
+        wsync();
+        reg_product<N, 0, 0>(rr, s, L.i, T);  // rho H_E
+        wsync();
+        slot_store_rows<N>(s, L.i, T);
+        wsync();
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const cd th = cconj(s[(L.i ^ c) * RS + c]);
+            const cd d = csub(th, T[c]);
+            dr[c] = mk(d.y, -d.x);
+        }
+    }
+    if (a.dephase) {
+        // dephasing in the eigenbasis: U^dagger (D o (U rho U^dagger)) U
+        cd T[C];
+        wsync();
+        slot_store_cols<N>(s, L.i, U);  // U -> S
+        wsync();
+        reg_product<N, 0, 1>(T1, s, L.i, T);  // rhoH = (U rho) U^dagger
+        const double di = Dg[L.kk * N + L.i];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const int j = L.i ^ c;
+            double gam;
+            if (a.w < 0) {
+                gam = (L.i != j) ? 1.0 : 0.0;
+            } else {
+                const double dE = (di - Dg[L.kk * N + j]) / a.w;
+                gam = 1 - exp(-dE * dE);
+            }
+            T[c] = cscale(T[c], -gam * a.F2);
+        }
+        reg_product<N, 0, 0>(T, s, L.i, T1);  // Q = rhoH' U
+        wsync();
+        slot_store_rows<N>(s, L.i, T1);  // Q -> S
+        wsync();
+        reg_product<N, 1, 0>(U, s, L.i, T);  // U^dagger Q
+#pragma unroll
+        for (int c = 0; c < C; c++) dr[c] = cadd(dr[c], T[c]);
+    }
+    // d rho/dt through LDS for coalescing
+    wsync();
+    slot_store_rows<N>(s, L.i, dr);
+    wsync();
+    store_state<N, 1, Lay>(a.dxdt, S, n, a.P, a.nk, blk);
+    if (a.err_partial) {
+        const double v = seg_max<64>(stage_err<N, 1, Lay>(a, S, blk));
+        if (threadIdx.x == 0) a.err_partial[blk] = v;
+    }
+}
+
 // the stage for the KPW k-points of logical block blk of one instance
 template <int N, int LR>
 __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
@@ -834,6 +1048,9 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     constexpr int C = G::C;
     if constexpr (N == 16) {
         rhs_stage_body16<LR>(a, blk);
+        return;
+    } else if constexpr (LR == 1) {
+        rhs_stage_body_r1<N>(a, blk);
         return;
     } else {
     __shared__ cd S0[G::KPW * G::NN];
@@ -1213,6 +1430,55 @@ static hipError_t launch_rhs_batched_NL(const StageArgs *args, const long *block
     return hipGetLastError();
 }
 
+// This file is compiled twice.  SBE_STAGE_PART 2 holds the one-lane-per-row
+// stage kernels (N = 2, 4, 8 at LR = 1), built with the default machine
+// scheduler: the ILP-oriented one takes them past the register count of
+// three waves per SIMD.  SBE_STAGE_PART 1 holds everything else and the
+// dispatchers, built with the ILP-oriented scheduler.  0 (the default)
+// builds both in one object.
+#ifndef SBE_STAGE_PART
+#define SBE_STAGE_PART 0
+#endif
+
+hipError_t launch_rhs_stage_r1(int N, const StageArgs &a, hipStream_t s);
+hipError_t launch_rhs_stage_set_r1(int N, const StageSet &ss, hipStream_t s);
+hipError_t launch_rhs_stage_batched_r1(int N, const StageArgs *args, const long *block0, int m, long blocks,
+                                       hipStream_t s);
+
+#if SBE_STAGE_PART != 1
+hipError_t launch_rhs_stage_r1(int N, const StageArgs &a, hipStream_t s)
+{
+    switch (N) {
+    case 2: return launch_rhs_NL<2, 1>(a, s);
+    case 4: return launch_rhs_NL<4, 1>(a, s);
+    case 8: return launch_rhs_NL<8, 1>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs_stage_set_r1(int N, const StageSet &ss, hipStream_t s)
+{
+    switch (N) {
+    case 2: return launch_rhs_set_NL<2, 1>(ss, s);
+    case 4: return launch_rhs_set_NL<4, 1>(ss, s);
+    case 8: return launch_rhs_set_NL<8, 1>(ss, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs_stage_batched_r1(int N, const StageArgs *args, const long *block0, int m, long blocks,
+                                       hipStream_t s)
+{
+    switch (N) {
+    case 2: return launch_rhs_batched_NL<2, 1>(args, block0, m, blocks, s);
+    case 4: return launch_rhs_batched_NL<4, 1>(args, block0, m, blocks, s);
+    case 8: return launch_rhs_batched_NL<8, 1>(args, block0, m, blocks, s);
+    }
+    return hipErrorInvalidValue;
+}
+#endif
+
+#if SBE_STAGE_PART != 2
 int stage_k_per_block(int N, int LR) { return 64 / (N * LR); }
 
 template <int N, int LR>
@@ -1239,12 +1505,12 @@ int stage_variants(int N, int *lr_out, int cap)
 hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s)
 {
     switch (N * 100 + LR) {
-    case 201: return launch_rhs_NL<2, 1>(a, s);
+    case 201: return launch_rhs_stage_r1(2, a, s);
     case 202: return launch_rhs_NL<2, 2>(a, s);
-    case 401: return launch_rhs_NL<4, 1>(a, s);
+    case 401: return launch_rhs_stage_r1(4, a, s);
     case 402: return launch_rhs_NL<4, 2>(a, s);
     case 404: return launch_rhs_NL<4, 4>(a, s);
-    case 801: return launch_rhs_NL<8, 1>(a, s);
+    case 801: return launch_rhs_stage_r1(8, a, s);
     case 802: return launch_rhs_NL<8, 2>(a, s);
     case 804: return launch_rhs_NL<8, 4>(a, s);
     case 1602: return launch_rhs_NL<16, 2>(a, s);
@@ -1256,12 +1522,12 @@ hipError_t launch_rhs_stage(int N, int LR, const StageArgs &a, hipStream_t s)
 hipError_t launch_rhs_stage_set(int N, int LR, const StageSet &ss, hipStream_t s)
 {
     switch (N * 100 + LR) {
-    case 201: return launch_rhs_set_NL<2, 1>(ss, s);
+    case 201: return launch_rhs_stage_set_r1(2, ss, s);
     case 202: return launch_rhs_set_NL<2, 2>(ss, s);
-    case 401: return launch_rhs_set_NL<4, 1>(ss, s);
+    case 401: return launch_rhs_stage_set_r1(4, ss, s);
     case 402: return launch_rhs_set_NL<4, 2>(ss, s);
     case 404: return launch_rhs_set_NL<4, 4>(ss, s);
-    case 801: return launch_rhs_set_NL<8, 1>(ss, s);
+    case 801: return launch_rhs_stage_set_r1(8, ss, s);
     case 802: return launch_rhs_set_NL<8, 2>(ss, s);
     case 804: return launch_rhs_set_NL<8, 4>(ss, s);
     case 1602: return launch_rhs_set_NL<16, 2>(ss, s);
@@ -1274,12 +1540,12 @@ hipError_t launch_rhs_stage_batched(int N, int LR, const StageArgs *args, const 
                                     hipStream_t s)
 {
     switch (N * 100 + LR) {
-    case 201: return launch_rhs_batched_NL<2, 1>(args, block0, m, blocks, s);
+    case 201: return launch_rhs_stage_batched_r1(2, args, block0, m, blocks, s);
     case 202: return launch_rhs_batched_NL<2, 2>(args, block0, m, blocks, s);
-    case 401: return launch_rhs_batched_NL<4, 1>(args, block0, m, blocks, s);
+    case 401: return launch_rhs_stage_batched_r1(4, args, block0, m, blocks, s);
     case 402: return launch_rhs_batched_NL<4, 2>(args, block0, m, blocks, s);
     case 404: return launch_rhs_batched_NL<4, 4>(args, block0, m, blocks, s);
-    case 801: return launch_rhs_batched_NL<8, 1>(args, block0, m, blocks, s);
+    case 801: return launch_rhs_stage_batched_r1(8, args, block0, m, blocks, s);
     case 802: return launch_rhs_batched_NL<8, 2>(args, block0, m, blocks, s);
     case 804: return launch_rhs_batched_NL<8, 4>(args, block0, m, blocks, s);
     case 1602: return launch_rhs_batched_NL<16, 2>(args, block0, m, blocks, s);
@@ -1322,5 +1588,7 @@ hipError_t launch_init_state(int N, int LR, const InitArgs &a, hipStream_t s)
     }
     return hipErrorInvalidValue;
 }
+
+#endif
 
 }  // namespace sbe
