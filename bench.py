@@ -314,6 +314,7 @@ def main():
                 r = measure(other, max(args.steps, 50), args.warmup, cpu_seconds=None)
                 out[other] = {k: r["line"][k] for k in ("value", "unit", "ms_per_step", "roofline", "config",
                                                          "breakdown_ms_per_rhs", "stage_tuning")}
+        out["chain1d_integrate"] = measure_integrate("chain1d")
     print(json.dumps(out), flush=True)
 
 
@@ -324,6 +325,40 @@ WORKLOADS = {
     "cubic3d": "cubic3d: seeded 3-D 16-band model, 729 R, 48^3 k, sin2 2 um 10 cycles 8 MV/cm, T2 8 fs, "
                "rtol 1e-8, Jacobi <= 5 sweeps (warm-started)",
 }
+
+
+def measure_integrate(name, repeats=3):
+    """a whole integrate_const (the model's pulse window and output count)
+    with the step-size control on the host and on the resident integrator
+    (one cooperative launch): wall time of the call, k-RHS/s of its RHS
+    evaluations, best of `repeats` after one untimed run each"""
+    import dataclasses
+    import time
+
+    from sbewh_amd import model
+    from sbewh_amd.propagator import SBEContext
+
+    cfg = model.config(name)
+    res = {}
+    for label, dev in (("host_controller", False), ("resident", True)):
+        ctx = SBEContext(dataclasses.replace(cfg, device_controller=dev))
+        x0 = ctx.initial_state(ctx.fermi_level(cfg.occupied_bands))
+        ctx.integrate(x0.copy(), cfg.output_count, log_capacity=0)
+        best, r = None, None
+        for _ in range(repeats):
+            x = x0.copy()
+            t0 = time.perf_counter()
+            r = ctx.integrate(x, cfg.output_count, log_capacity=0)
+            dt = time.perf_counter() - t0
+            best = dt if best is None or dt < best else best
+        res[label] = {"seconds": best, "steps": r["steps"], "rhs_calls": r["rhs_calls"],
+                      "k_rhs_per_s": ctx.num_k * r["rhs_calls"] / best, "device_controller": r["device_controller"],
+                      "outputs": cfg.output_count}
+        ctx.close()
+    res["resident_over_host"] = res["host_controller"]["seconds"] / res["resident"]["seconds"]
+    print(f"[bench] {name} integrate_const: host {res['host_controller']['seconds'] * 1e3:.1f} ms, resident "
+          f"{res['resident']['seconds'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    return res
 
 
 def measure(name, steps, warmup, cpu_seconds):

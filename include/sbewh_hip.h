@@ -127,6 +127,10 @@ int sbe_observe_device_noalloc(sbe_ctx *ctx, const double *d_state, double *d_j,
  * receives the number of RHS evaluations. */
 long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out, double *j_out,
                    sbe_steplog *log, const sbe_steplog *replay, long *rhs_calls);
+/* 1 if the last sbe_integrate ran on the resident integrator
+ * (SBE_FLAG_DEVICE_CONTROLLER), 0 if on the host controller.  No
+ * counterpart in the reference (whose controller is Boost.Odeint on the host). */
+int sbe_integrate_info(const sbe_ctx *ctx, int *device_controller);
 
 /* ---- full observer ------------------------------------------------------
  * ObserverCPU with k-space regions, occupations and the region-averaged

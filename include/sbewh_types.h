@@ -75,7 +75,14 @@ enum {
      * GPUstate).  Without the flag: the CPU layout with each rho_k padded to
      * 8 * ceil(n*n / 8) complex (code/src/StateContext.h:7-25).  Both layouts
      * are identical when n*n is a multiple of 8. */
-    SBE_FLAG_GPU_LAYOUT = 2
+    SBE_FLAG_GPU_LAYOUT = 2,
+    /* sbe_integrate without a step log, replay or observer on a 1-D instance
+     * of at most 8 bands whose stage blocks can all be resident: the whole
+     * integrate_const in one cooperative launch, the step-size control on the
+     * device (sbe_integrate_info reports whether it ran).  Same step rules;
+     * the device's pow / cos / sin may round differently from the host's, so
+     * the step sequence can differ at that level from the host controller's. */
+    SBE_FLAG_DEVICE_CONTROLLER = 4
 };
 
 enum sbe_pulse_type {

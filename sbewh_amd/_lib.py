@@ -22,7 +22,7 @@ EXPORTS = ("sbe_version", "sbe_last_error", "sbe_create", "sbe_destroy", "sbe_se
            "sbe_scan_destroy", "sbe_scan_last_error", "sbe_scan_run", "sbe_scan_state_size", "sbe_scan_final_state",
            "sbe_scan_tuning", "sbe_observer_create", "sbe_observer_destroy", "sbe_observer_reset",
            "sbe_observer_record", "sbe_observer_record_device", "sbe_observer_count", "sbe_observer_results",
-           "sbe_integrate_observed", "sbe_observe_device_noalloc", "sbe_direct_omega", "sbe_direct_spectrum",
+           "sbe_integrate_observed", "sbe_integrate_info", "sbe_observe_device_noalloc", "sbe_direct_omega", "sbe_direct_spectrum",
            "sbe_hhg_spectrum_direct", "sbe_smear_spectrum")
 # every symbol include/sbewh_surrogate.h declares
 SURROGATE_EXPORTS = ("sbe_mlp_create", "sbe_mlp_destroy", "sbe_mlp_last_error", "sbe_mlp_param_count",
@@ -99,6 +99,7 @@ def load():
     L.sbe_observer_record_device.argtypes = [vp, vp, d, vp]
     L.sbe_observer_count.argtypes = [vp, P(C.c_int)]
     L.sbe_observer_results.argtypes = [vp, P(SbeObservations)]
+    L.sbe_integrate_info.argtypes = [vp, P(i)]
     L.sbe_integrate_observed.restype = l
     L.sbe_integrate_observed.argtypes = [vp, dp, i, vp, dp, dp, P(SbeSteplog), P(SbeSteplog), P(C.c_long)]
     fp = P(C.c_float)

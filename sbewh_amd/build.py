@@ -24,7 +24,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{SRC}"]
 SOURCES = ["phase_kernels.hip", "stage_kernels.hip", "rk_kernels.hip", "observer_kernels.hip", "sbe_api.cpp",
            "observer_api.cpp", "scan_api.cpp", "spectrum.cpp", "mlp_kernels.hip", "mlp_api.cpp", "noalloc.hip",
-           "w90.cpp"]
+           "w90.cpp", "resident_kernels.hip"]
 
 
 def _local_includes(path: str, seen=None) -> list:

@@ -19,6 +19,8 @@
 
 #include "../../include/sbewh_hip.h"
 #include "kernels.hpp"
+#include "dopri5.hpp"
+#include "pulse.hpp"
 
 using namespace sbe;
 
@@ -52,58 +54,6 @@ struct ArgFail {
     throw ArgFail{code, buf};
 }
 
-// ---- pulses: code/src/pulse/{Pulse3D,Sin2Pulse1D,GaussianPulse1D,PureGauss1D,Sin2RampPulse1D}.cpp
-struct SubPulse {
-    sbe_subpulse p;
-    double pol[3];
-};
-
-inline double sin2_env_field(double E0, double w, double cep, double beta, double t)
-{
-    double cp = w * t + cep, ec = std::cos(beta * t);
-    return E0 * (std::cos(cp) * ec * ec - 2 * beta / w * std::sin(cp) * ec * std::sin(beta * t));
-}
-
-inline double sub_E(const sbe_subpulse &p, double t)
-{
-    const double w = p.omega, F = p.fwhm, E0 = p.E0;
-    switch (p.type) {
-    case SBE_PULSE_SIN2:
-        if (t < -F || t > F) return 0;
-        return sin2_env_field(E0, w, p.cep, M_PI / 2 / F, t);
-    case SBE_PULSE_GAUSS: {
-        double beta = 4 * std::log(2.0) / (F * F), cp = w * t + p.cep;
-        return E0 * (-2 * beta / w * t * std::sin(cp) + std::cos(cp)) * std::exp(-beta * t * t);
-    }
-    case SBE_PULSE_PURE_GAUSS: {
-        double beta = 4 * std::log(2.0) / (F * F);
-        return E0 * std::exp(-beta * t * t);
-    }
-    case SBE_PULSE_SIN2_RAMP: {
-        double rc = p.rising_cycles, beta = w / 4 / rc;
-        int co = p.cycles_on;
-        double rising = 2 * M_PI / w * rc, on = M_PI / w * co, half = rising + M_PI / w * co;
-        if (t < -half) return 0;
-        if (t < -on) return sin2_env_field(E0, w, 0.0, beta, t + on);
-        if (t < on) return ((co % 2 == 0) ? 1 : -1) * E0 * std::cos(w * t);
-        if (t < half) return sin2_env_field(E0, w, 0.0, beta, t - on);
-        return 0;
-    }
-    }
-    return 0;
-}
-
-inline double sub_start(const sbe_subpulse &p)
-{
-    if (p.type == SBE_PULSE_SIN2_RAMP) return -2 * M_PI / p.omega * (p.rising_cycles + 0.5 * p.cycles_on);
-    return p.t_start_fwhm * p.fwhm;
-}
-inline double sub_end(const sbe_subpulse &p)
-{
-    if (p.type == SBE_PULSE_SIN2_RAMP) return 2 * M_PI / p.omega * (p.rising_cycles + 0.5 * p.cycles_on);
-    return p.t_end_fwhm * p.fwhm;
-}
-
 inline int smooth5(int N)
 {
     int n = N - 1, r;
@@ -114,18 +64,6 @@ inline int smooth5(int N)
     } while (r != 1);
     return n;
 }
-
-// Dormand-Prince 5(4) tableau (runge_kutta_dopri5)
-constexpr double A2 = 1.0 / 5, A3 = 3.0 / 10, A4 = 4.0 / 5, A5 = 8.0 / 9;
-constexpr double B21 = 1.0 / 5;
-constexpr double B31 = 3.0 / 40, B32 = 9.0 / 40;
-constexpr double B41 = 44.0 / 45, B42 = -56.0 / 15, B43 = 32.0 / 9;
-constexpr double B51 = 19372.0 / 6561, B52 = -25360.0 / 2187, B53 = 64448.0 / 6561, B54 = -212.0 / 729;
-constexpr double B61 = 9017.0 / 3168, B62 = -355.0 / 33, B63 = 46732.0 / 5247, B64 = 49.0 / 176,
-             B65 = -5103.0 / 18656;
-constexpr double C1 = 35.0 / 384, C3 = 500.0 / 1113, C4 = 125.0 / 192, C5 = -2187.0 / 6784, C6 = 11.0 / 84;
-constexpr double DC1 = C1 - 5179.0 / 57600, DC3 = C3 - 7571.0 / 16695, DC4 = C4 - 393.0 / 640,
-             DC5 = C5 - -92097.0 / 339200, DC6 = C6 - 187.0 / 2100, DC7 = -1.0 / 40;
 
 }  // namespace sbe_host
 
@@ -163,6 +101,7 @@ struct sbe_ctx {
     long fuse_below_blocks = 512;  // stage grids smaller than this run an attempt's stages in one launch
     long set_min_blocks = 512;  // block target of a set's contraction launch (best on the 150 x 150 2-D model)
     bool ucache_valid = false;
+    bool resident_used = false;  // the last integration ran on the resident integrator
     double *err_partial = nullptr;
     unsigned *err_ticket = nullptr;              // ticket of the fused attempt's last block (zero between launches)
     unsigned long long *err_host = nullptr;      // pinned, mapped, fine-grained
@@ -249,14 +188,7 @@ struct sbe_ctx {
         }
     }
 
-    void E_at(double t, double E[3]) const
-    {
-        E[0] = E[1] = E[2] = 0;
-        for (const auto &s : pulses) {
-            double e = sub_E(s.p, t - s.p.center);
-            for (int d = 0; d < 3; d++) E[d] += e * s.pol[d];
-        }
-    }
+    void E_at(double t, double E[3]) const { pulse_E(pulses.data(), (int)pulses.size(), t, E); }
     double t_start() const
     {
         double r = INFINITY;
@@ -612,7 +544,8 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
     if (m->num_cells < 1 || !m->cells || !m->H || !m->D) fail(SBE_ERR_ARG, "empty model");
     if (g->dim < 1 || g->dim > 3) fail(SBE_ERR_ARG, "dim must be 1, 2 or 3");
     if (prm->sweep_count < 0 || prm->sweep_count > 50) fail(SBE_ERR_ARG, "sweep_count must be in [0, 50]");
-    if (prm->flags & ~(SBE_FLAG_COLD_JACOBI | SBE_FLAG_GPU_LAYOUT)) fail(SBE_ERR_ARG, "unknown flags 0x%x", prm->flags);
+    if (prm->flags & ~(SBE_FLAG_COLD_JACOBI | SBE_FLAG_GPU_LAYOUT | SBE_FLAG_DEVICE_CONTROLLER))
+        fail(SBE_ERR_ARG, "unknown flags 0x%x", prm->flags);
     const int n = m->num_wann, nn = n * n;
     c->n = n;
     c->nn = nn;
@@ -800,8 +733,6 @@ inline void build(sbe_ctx *c, const sbe_model *m, const sbe_grid *g, const sbe_p
 // Fermi-Dirac ground state rho_k = f(H0(k)) of the context into dst (device)
 void initial_state_dev(sbe_ctx &c, double mu, double *dst, hipStream_t s);
 
-inline bool less_with_sign(double t1, double t2) { return t2 - t1 > 2.220446049250313e-16; }
-inline bool less_eq_with_sign(double t1, double t2) { return t1 - t2 <= 2.220446049250313e-16; }
 
 inline void log_push(sbe_steplog *log, double t, double dt, double err, int acc)
 {

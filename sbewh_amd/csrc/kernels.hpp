@@ -307,4 +307,47 @@ struct RegionArgs {
 };
 hipError_t launch_region_sums(const RegionArgs &a, hipStream_t s);
 
+// ---- resident integrator (resident_kernels.hip) ---------------------------
+// The whole integrate_const of a 1-D single instance in one cooperative
+// launch: each block owns the stage kernel's k-points of its index, runs the
+// six stages of every attempt on them, and every block evaluates the same
+// step-size control from the same error norm (a grid-wide barrier after the
+// phase-sum preparation and after the error terms).  Output times as the host
+// controller: j(t) at t0 + i dt_out into j_out, the fresh derivative at each
+// interval start, steps truncated onto the output grid.
+struct SubPulse;
+struct ResidentArgs {
+    PhaseGeom g;
+    const double *Hb, *Db;     // dense model box [W][nn], position operator [3][W][nn]
+    long nbox;
+    int n, nn, P, dim;
+    long nk;
+    double *x, *xn;            // state and candidate (swapped on acceptance)
+    double *k[8];              // k[1] .. k[7] (k[1], k[7] swapped on acceptance: FSAL)
+    double *tw6;               // SET_MAX twiddle sets (each block writes the rows of its k-points)
+    double *Yb;                // merged boxes, one SET_MAX group per block: [block][SET_MAX][Y_stride]
+    long tw_stride, Y_stride;
+    const int *kphase;
+    int WL;
+    long twL_off;
+    double *ucache;
+    double *err_partial;       // [blocks]
+    const double *Yobs;        // observer operators [W][(1 + 2 dim) nn]
+    double *jpartial;          // [nk][3]
+    double *j_out, *t_out;     // [output_count][3], [output_count]
+    const SubPulse *pulses;
+    int npulse;
+    double F2, w, eps_abs, eps_rel;
+    int sweeps, cold_sweeps, cold_always;
+    double t0, t1;
+    int output_count;
+    long max_attempts;
+    long *status;              // [0] accepted steps, [1] RHS evaluations, [2] attempts, [3] code, [4] x / xn parity
+};
+// status[3]: 0 done, 1 NaN error norm, 2 no step size within 500 tries,
+// 3 attempt limit reached.  Returns hipErrorCooperativeLaunchTooLarge when
+// the grid cannot be co-resident.
+hipError_t launch_resident(int N, int LR, const ResidentArgs &r, hipStream_t s);
+int resident_max_blocks(int N, int LR);
+
 }  // namespace sbe

@@ -218,7 +218,113 @@ long sbe_integrate(sbe_ctx *ctx, double *state, int output_count, double *t_out,
     return integrate_impl(ctx, state, output_count, t_out, j_out, log, replay, rhs_calls, nullptr);
 }
 
+int sbe_integrate_info(const sbe_ctx *ctx, int *device_controller)
+{
+    if (!ctx || !device_controller) return SBE_ERR_ARG;
+    *device_controller = ctx->resident_used ? 1 : 0;
+    return SBE_OK;
+}
+
 }  // extern "C"
+
+namespace {
+
+// integrate_const on the resident integrator (one cooperative launch, the
+// step-size control on the device); false when the instance does not qualify
+// (not 1-D, more than 8 bands, or a grid whose blocks cannot all be resident)
+bool resident_integrate(sbe_ctx &c, int output_count, std::vector<double> &tv, long &steps)
+{
+    // (1-D, at most 8 bands, an R extent of at most 32 cells)
+    if (c.dim != 1 || c.N > 8 || c.geom.W[0] > 32 || c.twL_off != 0) return false;
+    if (!c.lr) {
+        // the layout tuner runs on a first stage (not part of the integration)
+        const long calls = c.rhs_calls;
+        c.rhs_dev(c.x(), c.st[1], c.t_start(), c.stream);
+        c.rhs_calls = calls;
+    }
+    const long kpb = stage_k_per_block(c.N, c.lr), blocks = (c.nk + kpb - 1) / kpb;
+    if (blocks > resident_max_blocks(c.N, c.lr)) return false;
+    // the observer operators do not depend on E: formed once
+    HIPCHK(launch_phase_partial(c.geom, c.pbufs(), c.obs_ops, 0, 0, 0, c.Yobs, nullptr, c.stream));
+    struct Buf {
+        void *p = nullptr;
+        ~Buf()
+        {
+            if (p) (void)hipFree(p);
+        }
+    } pulses, tdev, status, boxes;
+    HIPCHK(hipMalloc(&pulses.p, sizeof(SubPulse) * std::max<size_t>(1, c.pulses.size())));
+    HIPCHK(hipMemcpyAsync(pulses.p, c.pulses.data(), sizeof(SubPulse) * c.pulses.size(), hipMemcpyHostToDevice,
+                          c.stream));
+    HIPCHK(hipMalloc(&tdev.p, sizeof(double) * output_count));
+    HIPCHK(hipMalloc(&status.p, sizeof(long) * 8));
+    HIPCHK(hipMalloc(&boxes.p, sizeof(double) * blocks * SET_MAX * c.Y_stride));
+    ResidentArgs r{};
+    r.g = c.geom;
+    r.Hb = c.Hb;
+    r.Db = c.Db;
+    r.nbox = (long)c.geom.W[0] * c.geom.W[1] * c.geom.W[2];
+    r.n = c.n;
+    r.nn = c.nn;
+    r.P = c.P;
+    r.dim = c.dim;
+    r.nk = c.nk;
+    r.x = c.st[0];
+    r.xn = c.st[9];
+    for (int i = 1; i <= 7; i++) r.k[i] = c.st[i];
+    r.tw6 = c.tw6;
+    r.Yb = static_cast<double *>(boxes.p);
+    r.tw_stride = c.tw_stride;
+    r.Y_stride = c.Y_stride;
+    r.kphase = c.kphase;
+    r.WL = c.WL;
+    r.twL_off = c.twL_off;
+    r.ucache = c.ucache;
+    r.err_partial = c.err_partial;
+    r.Yobs = c.Yobs;
+    r.jpartial = c.partial;
+    r.j_out = c.jbuf;
+    r.t_out = static_cast<double *>(tdev.p);
+    r.pulses = static_cast<const SubPulse *>(pulses.p);
+    r.npulse = (int)c.pulses.size();
+    r.F2 = c.prm.relaxation_F2;
+    r.w = c.prm.soothing_width;
+    r.eps_abs = c.prm.eps_abs;
+    r.eps_rel = c.prm.eps_rel;
+    r.sweeps = c.sweeps_for(1);
+    r.cold_sweeps = c.sweeps_for(0);
+    r.cold_always = (c.prm.flags & SBE_FLAG_COLD_JACOBI) ? 1 : 0;
+    r.t0 = c.t_start();
+    r.t1 = c.t_end();
+    r.output_count = output_count;
+    r.max_attempts = 100000000L;
+    r.status = static_cast<long *>(status.p);
+    const hipError_t e = launch_resident(c.N, c.lr, r, c.stream);
+    if (e == hipErrorCooperativeLaunchTooLarge) {
+        (void)hipGetLastError();
+        return false;
+    }
+    HIPCHK(e);
+    long st[8] = {};
+    tv.assign(output_count, 0.0);
+    HIPCHK(hipMemcpyAsync(st, status.p, sizeof st, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(tv.data(), tdev.p, sizeof(double) * output_count, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (st[4]) {
+        std::swap(c.st[0], c.st[9]);
+        std::swap(c.st[1], c.st[7]);
+    }
+    c.rhs_calls += st[1];
+    if (c.prm.relaxation_F2 > 0) c.ucache_valid = true;
+    if (st[3] == 1) fail(SBE_ERR_STEP, "error norm is NaN (resident integrator, attempt %ld)", st[2]);
+    if (st[3] == 2) fail(SBE_ERR_STEP, "Max number of iterations exceeded (500). A new step size was not found.");
+    if (st[3] == 3) fail(SBE_ERR_STEP, "resident integrator: attempt limit %ld reached", r.max_attempts);
+    steps = st[0];
+    c.resident_used = true;
+    return true;
+}
+
+}  // namespace
 
 long sbe_host::integrate_impl(sbe_ctx *ctx, double *state, int output_count, double *t_out, double *j_out,
                               sbe_steplog *log, const sbe_steplog *replay, long *rhs_calls, sbe_observer *obs)
@@ -243,6 +349,19 @@ long sbe_host::integrate_impl(sbe_ctx *ctx, double *state, int output_count, dou
         HIPCHK(hipMemsetAsync(c.err_ticket, 0, sizeof(unsigned), c.stream));
         if (log) log->count = 0;
         HIPCHK(hipMemcpyAsync(c.x(), state, 16 * c.S, hipMemcpyHostToDevice, c.stream));
+        c.resident_used = false;
+        std::vector<double> tres;
+        if ((c.prm.flags & SBE_FLAG_DEVICE_CONTROLLER) && !replay && !log && !obs &&
+            resident_integrate(c, output_count, tres, steps)) {
+            HIPCHK(hipMemcpyAsync(state, c.x(), 16 * c.S, hipMemcpyDeviceToHost, c.stream));
+            std::vector<double> jh(3 * (size_t)output_count, 0.0);
+            HIPCHK(hipMemcpyAsync(jh.data(), c.jbuf, 3 * sizeof(double) * output_count, hipMemcpyDeviceToHost,
+                                  c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            if (j_out) std::memcpy(j_out, jh.data(), 3 * sizeof(double) * output_count);
+            if (t_out) std::memcpy(t_out, tres.data(), sizeof(double) * output_count);
+            return;
+        }
         const double t0 = c.t_start(), t1 = c.t_end();
         const double dt_out = (t1 - t0) / (output_count - 1);
         const double t_end = t1 + dt_out / 2;

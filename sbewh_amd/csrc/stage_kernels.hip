@@ -1,1356 +1,8 @@
-// Fused SBE right-hand-side stage for CDNA4 (gfx950), FP64.
-//
-// Per k-point:  d rho/dt = -i [H_E, rho]  +  U^dagger (D o (U rho U^dagger)) U
-// where U H_0 U^dagger = diag(eps) comes from a wavefront-level complex Jacobi
-// eigensolver and D_ab = -F2 (1 - exp(-((eps_a - eps_b)/w)^2)).
-// Reference semantics: code/src/PropagatorCPU.cpp:73-155,208-256 (RHS),
-// code/src/util/DiagonalizatorSingle.cpp:57-122 (rotation formula, threshold).
-//
-// Data layout in registers ("xrow"): the k-point's N x N matrix is spread over
-// LK = N*LR lanes; lane (i, h) owns row i, columns j = i ^ (h*C + c) for its
-// C = N/LR registers c.  With this XOR-wrapped layout and the XOR pairing
-// (p, p ^ r) of Jacobi round r, every register index in the rotation and
-// product loops is a compile-time constant and every cross-lane exchange is
-// a fixed lane-xor.  One sweep = rounds r = 1..N-1 (each unordered pair once).
-// The reference's cyclic order and this parallel order converge to the same
-// decomposition; the dephasing term depends only on the converged
-// eigen-decomposition (invariant under eigenvector phases and under
-// rotations within degenerate subspaces, where gamma -> 0).
-//
-// LR (lanes per row) trades registers per lane against lanes per k-point; the
-// host tunes it per problem (one wavefront holds KPW = 64/(N LR) k-points).
-#include <utility>
-
-#include "device_common.hpp"
-#include "kernels.hpp"
+// Launchers of the fused SBE right-hand-side stage kernels (device code in
+// stage_device.hpp).
+#include "stage_device.hpp"
 
 namespace sbe {
-
-template <int N, int LR>
-struct Geo {
-    static_assert(N * LR <= 64 && LR <= N, "layout must fit one wavefront");
-    static constexpr int C = N / LR;     // matrix entries per lane
-    static constexpr int LK = N * LR;    // lanes per k-point
-    static constexpr int KPW = 64 / LK;  // k-points per wavefront
-    static constexpr int NN = N * N;
-};
-
-struct Lane {
-    int kk, i, h;
-    long k;
-    bool valid;
-};
-
-template <int N, int LR>
-__device__ __forceinline__ Lane lane_id(long nk, long blk)
-{
-    using G = Geo<N, LR>;
-    Lane L;
-    int lane = threadIdx.x;
-    L.kk = lane / G::LK;
-    int li = lane % G::LK;
-    L.i = li / LR;
-    L.h = li % LR;
-    L.k = blk * G::KPW + L.kk;
-    L.valid = L.k < nk;
-    return L;
-}
-
-// LDS position of element (a, b) of an N x N matrix: row a, column b ^ g(a),
-// with g(a) = a * x in GF(2^m) (N = 2^m; x^2+x+1, x^3+x+1, x^4+x+1).  g and
-// a -> a ^ g(a) are bijections, so the wavefront's column walks (X[i][l] over
-// rows i), row walks (Y[l][j]) and XOR-wrapped row accesses (row i, column
-// i ^ c) of one k-point all fall on distinct LDS banks.
-template <int N>
-__device__ __forceinline__ int sw(int a, int b)
-{
-    if constexpr (N == 2) {
-        return a * N + b;
-    } else {
-        // reduction by x^m = x + 1: the same low bits (3) for m = 2, 3, 4
-        const int g = ((a << 1) & (N - 1)) ^ ((a & (N >> 1)) ? 3 : 0);
-        return a * N + (b ^ g);
-    }
-}
-
-// LDS layouts of a k-point's N x N matrix (MS entries per k-point):
-// SwLay is the swizzled natural layout above; SlotLay keeps element (a, b) in
-// row a, slot a ^ b, with a row stride of N + 1 entries.  In SlotLay the
-// entries a lane of the one-lane-per-row layout touches in a product,
-// Y[i ^ c'][i ^ c] (slot c' ^ c) and its transposed reads (row i ^ c, slot c),
-// sit at compile-time offsets from one row address, and the odd row stride
-// spreads the rows of a k-point over the LDS banks.
-template <int N>
-struct SwLay {
-    static constexpr int MS = N * N;
-    static constexpr int CHUNK = 64;  // entries per lane whose global loads are in flight together
-    __device__ static int at(int a, int b) { return sw<N>(a, b); }
-};
-template <int N>
-struct SlotLay {
-    static constexpr int RS = N + 1;
-    static constexpr int MS = N * RS;
-    static constexpr int CHUNK = 2;
-    __device__ static int at(int a, int b) { return a * RS + (a ^ b); }
-};
-
-// f(integral_constant<int, c * CH>) for c = 0 .. IT/CH - 1, unrolled at compile
-// time (arrays indexed by the chunk offset stay in registers)
-template <int CH, class F, int... I>
-__device__ __forceinline__ void chunks_impl(F &&f, std::integer_sequence<int, I...>)
-{
-    (f(std::integral_constant<int, I * CH>{}), ...);
-}
-template <int IT, int CH, class F>
-__device__ __forceinline__ void for_chunks(F &&f)
-{
-    chunks_impl<CH>(f, std::make_integer_sequence<int, IT / CH>{});
-}
-
-// wave-local sync: blocks are exactly one wavefront
-__device__ __forceinline__ void wsync() { __syncthreads(); }
-
-// coalesced copy of the wave's KPW matrices (n x n at src + k*stride + off,
-// complex) into LDS natural layout [kk][N][N], zero padded.
-template <int N, int LR, class Lay = SwLay<N>>
-__device__ __forceinline__ void load_mats(cd *lds, const double *src, long stride, long off, int n, long nk, long blk)
-{
-    using G = Geo<N, LR>;
-    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
-    constexpr int CH = N >= 16 ? 4 : (Lay::CHUNK < IT ? Lay::CHUNK : IT);     // entries whose loads are in flight together
-    const double2 *s2 = reinterpret_cast<const double2 *>(src);
-    // unconditional loads (masked entries read entry `off` of k-point 0)
-    for_chunks<IT, CH>([&](auto i0c) {
-        constexpr int i0 = decltype(i0c)::value;
-        double2 v[CH];
-        bool ok[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int e = threadIdx.x + 64 * (i0 + u);
-            const int kk = e / G::NN, idx = e % G::NN;
-            const int a = idx / N, b = idx % N;
-            const long k = blk * G::KPW + kk;
-            ok[u] = a < n && b < n && k < nk;
-            v[u] = s2[ok[u] ? k * stride + off + a * n + b : off];
-        }
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int e = threadIdx.x + 64 * (i0 + u);
-            const int idx = e % G::NN;
-            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? mk(v[u].x, v[u].y) : mk(0, 0);
-        }
-    });
-}
-
-// H(k) of operator op (0: H0, 1: H_E) for the wave's k-points into LDS
-// natural layout: the last phase-sum contraction
-//   H(k)[a][b] = sum_r tw_L[i_L(k)][r] Y[rest(k)][r][op][a][b]
-// (the k-points of a wave share rest(k), so the Y rows are L1/L2 hits).
-template <int N, int LR, class Lay = SwLay<N>>
-__device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, long blk)
-{
-    using G = Geo<N, LR>;
-    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
-    constexpr int WQ = 16;                   // largest R extent of the last axis
-    const int n = a.n, nn = n * n, M = 2 * nn, WL = a.ph.WL;
-    if (WL > WQ) {
-        // wide R box along the last axis: one entry at a time
-        for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
-            const int kk = e / G::NN, idx = e % G::NN;
-            const int r = idx / N, c = idx % N;
-            const long k = blk * G::KPW + kk;
-            cd v = mk(0, 0);
-            if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, M, op * nn + r * n + c);
-            lds[kk * Lay::MS + Lay::at(r, c)] = v;
-        }
-        return;
-    }
-    const int2 *kph = reinterpret_cast<const int2 *>(a.ph.kphase);
-    const double2 *Yb = reinterpret_cast<const double2 *>(a.ph.Y);
-    const double2 *Tb = reinterpret_cast<const double2 *>(a.ph.twL);
-    // the lane's entries in chunks of CH, all r of a chunk in one unrolled
-    // block, so that the loads of different entries are in flight together
-    constexpr int CH = N >= 16 ? 4 : (Lay::CHUNK < IT ? Lay::CHUNK : IT);
-    for_chunks<IT, CH>([&](auto i0c) {
-        constexpr int i0 = decltype(i0c)::value;
-        cd acc[CH];
-        const double2 *yp[CH];
-        const double2 *tp[CH];
-        bool ok[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int e = threadIdx.x + 64 * (i0 + u);
-            const int kk = e / G::NN, idx = e % G::NN;
-            const int r = idx / N, c = idx % N;
-            const long k = blk * G::KPW + kk;
-            // masked entries read valid addresses and are zeroed at the end:
-            // the loads stay unconditional.  The k of an entry is uniform over
-            // its lanes when NN >= 64, so is its phase map.
-            const bool krc = r < n && c < n;
-            ok[u] = krc && k < a.nk;
-            const int2 ph = kph[k < a.nk ? k : 0];
-            yp[u] = Yb + (long)ph.x * M + (krc ? op * nn + r * n + c : 0);
-            tp[u] = Tb + ph.y;
-            acc[u] = mk(0, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < WQ; q++) {
-            if (q < WL) {
-#pragma unroll
-                for (int u = 0; u < CH; u++) {
-                    const double2 tq = tp[u][q], yq = yp[u][(long)q * M];
-                    acc[u] = cfma(mk(tq.x, tq.y), mk(yq.x, yq.y), acc[u]);
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int e = threadIdx.x + 64 * (i0 + u);
-            const int idx = e % G::NN;
-            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? acc[u] : mk(0, 0);
-        }
-    });
-}
-
-// stage input rho = x + sum_j c_j k_j (the Dormand-Prince stage combination,
-// left to right as in the reference's scale_sum) into LDS natural layout
-template <int N, int LR, class Lay = SwLay<N>>
-__device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
-{
-    using G = Geo<N, LR>;
-    const int n = a.n;
-    const StageInput &in = a.in;
-    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
-    constexpr int CH = N >= 16 ? 4 : (Lay::CHUNK < IT ? Lay::CHUNK : IT);     // entries whose loads are in flight together
-    for_chunks<IT, CH>([&](auto i0c) {
-        constexpr int i0 = decltype(i0c)::value;
-        long idx_[CH];
-        bool ok[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int e = threadIdx.x + 64 * (i0 + u);
-            const int kk = e / G::NN, idx = e % G::NN;
-            const int r = idx / N, c = idx % N;
-            const long k = blk * G::KPW + kk;
-            ok[u] = r < n && c < n && k < a.nk;
-            // masked entries read state entry 8 (valid) and are zeroed at the end
-            idx_[u] = ok[u] ? 8 + k * a.P + r * n + c : 8;
-        }
-        double re[CH], im[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const double2 xv = reinterpret_cast<const double2 *>(in.x)[idx_[u]];
-            re[u] = xv.x;
-            im[u] = xv.y;
-        }
-#pragma unroll
-        for (int j = 0; j < 6; j++)
-            if (j < in.nt) {
-                const double2 *kj = reinterpret_cast<const double2 *>(in.k[j]);
-                const double cj = in.c[j];
-#pragma unroll
-                for (int u = 0; u < CH; u++) {
-                    const double2 kv = kj[idx_[u]];
-                    re[u] += cj * kv.x;
-                    im[u] += cj * kv.y;
-                }
-            }
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int e = threadIdx.x + 64 * (i0 + u);
-            const int idx = e % G::NN;
-            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? mk(re[u], im[u]) : mk(0, 0);
-        }
-    });
-}
-
-// the wave's matrices from LDS natural layout into a state vector (reference
-// layout, zero pads), coalesced
-template <int N, int LR, class Lay = SwLay<N>>
-__device__ __forceinline__ void store_state(double *dst, const cd *lds, int n, int P, long nk, long blk)
-{
-    using G = Geo<N, LR>;
-    const int nn = n * n;
-    const long k0 = blk * G::KPW;
-    for (int e = threadIdx.x; e < G::KPW * P; e += 64) {
-        int kk = e / P, p = e % P;
-        long k = k0 + kk;
-        if (k >= nk) break;
-        cd v = mk(0, 0);
-        if (p < nn) v = lds[kk * Lay::MS + Lay::at(p / n, p % n)];
-        reinterpret_cast<double2 *>(dst)[8 + k * P + p] = make_double2(v.x, v.y);
-    }
-}
-
-// out[c] = sum_l X[i][l] Y[l][j_c] with X, Y in LDS (natural layout);
-// XOP 1 uses X[i][l] := conj(Xs[l][i]); YOP 1 uses Y[l][j] := conj(Ys[j][l]).
-template <int N, int LR, int XOP, int YOP>
-__device__ __forceinline__ void lds_product(const cd *X, const cd *Y, const Lane &L, cd *out)
-{
-    using G = Geo<N, LR>;
-#pragma unroll
-    for (int c = 0; c < G::C; c++) out[c] = mk(0, 0);
-#pragma unroll 2
-    for (int l = 0; l < N; l++) {
-        cd x = (XOP == 0) ? X[sw<N>(L.i, l)] : cconj(X[sw<N>(l, L.i)]);
-#pragma unroll
-        for (int c = 0; c < G::C; c++) {
-            int j = L.i ^ (L.h * G::C + c);
-            cd y = (YOP == 0) ? Y[sw<N>(l, j)] : cconj(Y[sw<N>(j, l)]);
-            out[c] = cfma(x, y, out[c]);
-        }
-    }
-}
-
-template <int N, int LR>
-__device__ __forceinline__ void store_xrow(cd *lds, const Lane &L, const cd *v)
-{
-    using G = Geo<N, LR>;
-#pragma unroll
-    for (int c = 0; c < G::C; c++) lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))] = v[c];
-}
-
-template <int N, int LR>
-__device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
-{
-    using G = Geo<N, LR>;
-#pragma unroll
-    for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))];
-}
-
-// The Jacobi's eigenvector registers.  With jac_ut, lane (i, h) register c
-// holds U[i ^ (h C + c)][i] (the xrow layout of U^T): a rotation of rows
-// (j, j ^ r) of U then combines register c with register c ^ (r mod C) of the
-// same lane (r < C) or of lane h ^ (r / C) in the row's own lane group, and it
-// uses the coefficients the column update of A reads anyway, instead of an
-// exchange with the partner row's lanes.  For 16 bands at 4 lanes per row
-// that trade would replace LDS permutes by more vector-ALU moves, so that
-// layout keeps U in rows.
-template <int N, int LR>
-constexpr bool jac_ut() { return !(N == 16 && LR == 4); }
-
-// U registers (either layout) <-> LDS natural layout
-template <int N, int LR>
-__device__ __forceinline__ void store_u(cd *lds, const Lane &L, const cd *v)
-{
-    using G = Geo<N, LR>;
-    if constexpr (jac_ut<N, LR>()) {
-#pragma unroll
-        for (int c = 0; c < G::C; c++) lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)] = v[c];
-    } else {
-        store_xrow<N, LR>(lds, L, v);
-    }
-}
-
-template <int N, int LR>
-__device__ __forceinline__ void load_u(const cd *lds, const Lane &L, cd *v)
-{
-    using G = Geo<N, LR>;
-    if constexpr (jac_ut<N, LR>()) {
-#pragma unroll
-        for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)];
-    } else {
-        load_xrow<N, LR>(lds, L, v);
-    }
-}
-
-// padded rows and columns (index >= n) of U are the identity
-template <int N, int LR>
-__device__ __forceinline__ void u_pad_identity(const Lane &L, int n, cd *v)
-{
-    using G = Geo<N, LR>;
-#pragma unroll
-    for (int c = 0; c < G::C; c++) {
-        const int g = L.h * G::C + c;
-        if (L.i >= n || (L.i ^ g) >= n) v[c] = mk(g == 0 ? 1.0 : 0.0, 0);
-    }
-}
-
-// U registers of the Jacobi -> xrow layout (rows), through the k-point's
-// N x N LDS region lds (free on entry, rewritten)
-template <int N, int LR>
-__device__ __forceinline__ void u_rows(cd *lds, const Lane &L, cd *v)
-{
-    if constexpr (jac_ut<N, LR>()) {
-        store_u<N, LR>(lds, L, v);
-        wsync();
-        load_xrow<N, LR>(lds, L, v);
-        wsync();
-    }
-}
-
-// ---- 16 x 16 complex products on the FP64 matrix cores (16-band layouts) --
-// v_mfma_f64_16x16x4f64: A (16 x 4) lane l holds A[l % 16][l / 16], B (4 x 16)
-// lane l holds B[l / 16][l % 16], the 16 x 16 accumulator lane l holds rows
-// 4 j + l / 16, j = 0..3, of column l % 16.  A complex product is four real
-// products of 16 x 16 x 16, each four MFMA steps along the inner index.
-typedef double d4 __attribute__((ext_vector_type(4)));
-struct CAcc {
-    d4 r, i;
-};
-
-__device__ __forceinline__ void cacc_zero(CAcc &c)
-{
-    c.r = d4{0, 0, 0, 0};
-    c.i = d4{0, 0, 0, 0};
-}
-
-// acc += SX * X Y for one k-point's matrices in LDS natural layout
-// (XOP 1: X[i][l] := conj(Xs[l][i]); YOP 1: Y[l][j] := conj(Ys[j][l]));
-// every operand is read before the first matrix instruction
-template <int XOP, int YOP, int SX = 1>
-__device__ __forceinline__ void mfma_cprod(const cd *X, const cd *Y, CAcc &acc)
-{
-    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-    cd a[4], b[4];
-#pragma unroll
-    for (int kc = 0; kc < 4; kc++) {
-        const int k = 4 * kc + q;
-        a[kc] = (XOP == 0) ? X[sw<16>(r, k)] : cconj(X[sw<16>(k, r)]);
-        b[kc] = (YOP == 0) ? Y[sw<16>(k, r)] : cconj(Y[sw<16>(r, k)]);
-        if (SX < 0) a[kc] = mk(-a[kc].x, -a[kc].y);
-    }
-#pragma unroll
-    for (int kc = 0; kc < 4; kc++) {
-        acc.r = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].x, b[kc].x, acc.r, 0, 0, 0);
-        acc.i = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].x, b[kc].y, acc.i, 0, 0, 0);
-        acc.r = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[kc].y, b[kc].y, acc.r, 0, 0, 0);
-        acc.i = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].y, b[kc].x, acc.i, 0, 0, 0);
-    }
-}
-
-// accumulator element j of this lane: row and column
-__device__ __forceinline__ int cacc_row(int j) { return 4 * j + ((threadIdx.x & 63) >> 4); }
-__device__ __forceinline__ int cacc_col() { return threadIdx.x & 15; }
-
-// the accumulator into LDS natural layout (one k-point)
-__device__ __forceinline__ void cacc_store(cd *lds, const CAcc &c)
-{
-#pragma unroll
-    for (int j = 0; j < 4; j++) lds[sw<16>(cacc_row(j), cacc_col())] = mk(c.r[j], c.i[j]);
-}
-
-__device__ __forceinline__ void cs_half(double y, double x, double &c, double &s)
-{
-    const double t = y / (fabs(x) + sqrt(fma(x, x, y * y)));
-    const double cc = rsqrt(fma(t, t, 1.0));
-    const double ss = t * cc;
-    if (x >= 0) {
-        c = cc;
-        s = ss;
-    } else {
-        c = fabs(ss);
-        s = copysign(cc, y);
-    }
-}
-
-// an optimisation barrier: the N accumulators are complete here, and no
-// memory access moves across it (keeps a product's loads per step bounded)
-template <int N>
-__device__ __forceinline__ void pin_rows(cd *v)
-{
-    if constexpr (N == 8) {
-        asm volatile("" : "+v"(v[0].x), "+v"(v[0].y), "+v"(v[1].x), "+v"(v[1].y), "+v"(v[2].x), "+v"(v[2].y),
-                     "+v"(v[3].x), "+v"(v[3].y), "+v"(v[4].x), "+v"(v[4].y), "+v"(v[5].x), "+v"(v[5].y),
-                     "+v"(v[6].x), "+v"(v[6].y), "+v"(v[7].x), "+v"(v[7].y)::"memory");
-    } else {
-#pragma unroll
-        for (int c = 0; c < N; c++) asm volatile("" : "+v"(v[c].x), "+v"(v[c].y)::"memory");
-    }
-}
-
-// One round r of the parallel Jacobi sweep: rotations on the pairs (p, q = p ^ r).
-// With a_pq = m w (m = |a_pq|, |w| = 1) the rotation of a pair is
-//   J = [[c, s w], [-s conj(w), c]],  (c, s) = (cos t, sin t), t = atan2(2m, a_pp - a_qq) / 2,
-// which zeroes (J A J^dagger)_pq; its diagonal is real.  It acts on rows,
-// row p <- c row p + s w row q and row q <- c row q - s conj(w) row p, and as
-// J^dagger on columns, column j <- J_jj column j + conj(J_j,j^r) column j ^ r.
-// Lane q takes a_pq from lane p, so both lanes of a pair compute the same
-// rotation from the same operands.  Each stores its row's coefficients
-// (J_jj, J_j,j^r) under index j (the identity when the pair does not
-// rotate), so every lane reads its row's and its columns' coefficients
-// without selects or divergent branches; an identity coefficient pair
-// reproduces its operand exactly.
-template <int N, int LR, int R>
-__device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double thr, double *Dg, cd *Ja, cd *Jb,
-                                          int *Fa, int &anyrot)
-{
-    using G = Geo<N, LR>;
-    constexpr int C = G::C;
-    constexpr int rc = R % C, rh = R / C;
-    constexpr int mrow = R * LR + rh;   // lane mask of the row partner (row i ^ R)
-    const int base = L.kk * N;
-    // (a) rotation of the pair (p, q = p ^ R).  Lanes h == rh of rows p and q
-    // hold elements (p, q) and (q, p) = conj(a_pq) in register rc.
-    {
-        constexpr int mpair = R * LR;
-        const bool isp = L.i < (L.i ^ R);
-        const cd e = A[rc];
-        const double thr2 = thr * thr;
-        // a round in which no pair of the wavefront rotates leaves A and U
-        // unchanged: skip it before computing any rotation (every exchange
-        // runs in all lanes: a DPP read from a lane that is switched off
-        // returns 0)
-        if (!__any(L.h == rh && isp && L.valid && fma(e.x, e.x, e.y * e.y) > thr2)) return;
-        anyrot = 1;
-        const cd o = xchg<mpair>(e);
-        const cd apq = isp ? e : o;
-        const int pr = isp ? L.i : (L.i ^ R);
-        if (L.h == rh) {
-            const double m2 = fma(apq.x, apq.x, apq.y * apq.y);
-            const int act = L.valid && (m2 > thr2);
-            double ja = 1, jbx = 0, jby = 0;
-            if (act) {
-                const double ri = rsqrt(m2);
-                double c, sn;
-                cs_half(2 * (m2 * ri), Dg[base + pr] - Dg[base + (pr ^ R)], c, sn);
-                const double swx = sn * (apq.x * ri), swy = sn * (apq.y * ri);
-                ja = c;
-                jbx = isp ? swx : -swx;
-                jby = swy;
-            }
-            Ja[base + L.i] = mk(ja, 0);
-            Jb[base + L.i] = mk(jbx, jby);
-            if (isp) Fa[base + L.i] = act;
-        }
-    }
-    wsync();
-    const int arow = Fa[base + min(L.i, L.i ^ R)];
-    const double ca = Ja[base + L.i].x;
-    const cd cb = Jb[base + L.i];
-#pragma unroll
-    for (int c = 0; c < C; c++) {
-        const int c2 = c ^ rc;
-        if (c2 < c) continue;
-        // row update of the register pair (c, c2); the partner row holds them in (c2, c)
-        cd a0 = A[c], a1 = A[c2];
-        const cd b0 = xchg<mrow>(A[c2]), b1 = xchg<mrow>(A[c]);
-        a0 = cfma(cb, b0, cscale(a0, ca));
-        if constexpr (!jac_ut<N, LR>()) {
-            const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
-            U[c] = cfma(cb, v0, cscale(U[c], ca));
-            if (c2 != c) U[c2] = cfma(cb, v1, cscale(U[c2], ca));
-        }
-        if (c2 != c) {
-            a1 = cfma(cb, b1, cscale(a1, ca));
-        } else {
-            a1 = a0;
-        }
-        // column update: partner of column j is column j ^ R, register c ^ rc
-        cd d0, d1;
-        if constexpr (rh == 0) {
-            d0 = a1;
-            d1 = a0;
-        } else {
-            d0 = xchg<rh>(a1);
-            d1 = xchg<rh>(a0);
-        }
-        const int j0 = L.i ^ (L.h * C + c);
-        const double ja = Ja[base + j0].x;
-        const cd jb = Jb[base + j0];
-        if constexpr (jac_ut<N, LR>()) {
-            // U^T registers: row j0 of U (register c) pairs with row j0 ^ R (register c2 of lane h ^ rh)
-            const cd u0 = U[c], u1 = U[c2];
-            cd e0, e1;
-            if constexpr (rh == 0) {
-                e0 = u1;
-                e1 = u0;
-            } else {
-                e0 = xchg<rh>(u1);
-                e1 = xchg<rh>(u0);
-            }
-            U[c] = cfma(jb, e0, cscale(u0, ja));
-            if (c2 != c) {
-                if constexpr (rh == 0) {
-                    // row j1 = j0 ^ R: (J_j1j1, J_j1j0) = (J_j0j0, -conj(J_j0j1))
-                    U[c2] = cfma(mk(-jb.x, jb.y), e1, cscale(u1, ja));
-                } else {
-                    const int j1 = L.i ^ (L.h * C + c2);
-                    U[c2] = cfma(Jb[base + j1], e1, cscale(u1, Ja[base + j1].x));
-                }
-            }
-        }
-        cd r0 = cfma_cj(jb, d0, cscale(a0, ja));
-        // the rotated pivot (i, i ^ R) is set to zero: it sits in register rc of lanes h == rh
-        if (c == rc && L.h == rh && arow) r0 = mk(0, 0);
-        A[c] = r0;
-        if (c2 != c) {
-            cd r1;
-            if constexpr (rh == 0) {
-                // column j1 = j0 ^ R is j0's partner: (J_j1j1, J_j1j0) = (J_j0j0, -conj(J_j0j1))
-                r1 = cfma(mk(-jb.x, -jb.y), d1, cscale(a1, ja));
-            } else {
-                // column j1 = i ^ (h C + c2) belongs to another pair (its partner is in lane h ^ rh)
-                const int j1 = L.i ^ (L.h * C + c2);
-                r1 = cfma_cj(Jb[base + j1], d1, cscale(a1, Ja[base + j1].x));
-            }
-            if (c2 == rc && L.h == rh && arow) r1 = mk(0, 0);
-            A[c2] = r1;
-        }
-    }
-    if (L.h == 0) Dg[base + L.i] = A[0].x;
-    wsync();
-}
-
-template <int N, int LR, int... Rm1>
-__device__ __forceinline__ void jac_sweep(std::integer_sequence<int, Rm1...>, cd *A, cd *U, const Lane &L, double thr,
-                                          double *Dg, cd *Ja, cd *Jb, int *Fa, int &anyrot)
-{
-    (jac_round<N, LR, Rm1 + 1>(A, U, L, thr, Dg, Ja, Jb, Fa, anyrot), ...);
-}
-
-// Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (rows), eigenvalues
-// in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Ja/Jb[KPW*N] (rotation coefficients), Fa[KPW*N].
-template <int N, int LR>
-__device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Ja, cd *Jb, int *Fa,
-                            bool initU = true)
-{
-    using G = Geo<N, LR>;
-    constexpr int C = G::C;
-    if (initU) {
-#pragma unroll
-        for (int c = 0; c < C; c++) U[c] = mk((L0.h * C + c) == 0 ? 1.0 : 0.0, 0);
-    }
-    if (L0.h == 0) Dg[L0.kk * N + L0.i] = A[0].x;
-    wsync();
-    for (int sw = 0; sw < sweeps; sw++) {
-        // re-materialise the lane coordinates each sweep so that the per-round
-        // LDS addresses are not hoisted out of the sweep loop (register pressure)
-        Lane L = L0;
-        asm volatile("" : "+v"(L.i), "+v"(L.h), "+v"(L.kk));
-        double s = 0;
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            int j = L.i ^ (L.h * C + c);
-            if (L.i <= j) s += fabs(A[c].x) + fabs(A[c].y);
-        }
-        s = seg_sum<G::LK>(s);
-        const double thr = 1e-15 * (0.5 * s);
-        int anyrot = 0;
-        jac_sweep<N, LR>(std::make_integer_sequence<int, N - 1>{}, A, U, L, thr, Dg, Ja, Jb, Fa, anyrot);
-        if (!__any(anyrot)) break;
-    }
-}
-
-// eigenvector rows from LDS natural layout into the per-k cache [k][n][n],
-// with the lane -> entry mapping of load_mats
-template <int N, int LR, class Lay = SwLay<N>>
-__device__ __forceinline__ void store_ucache(const StageArgs &a, const cd *lds, long blk)
-{
-    using G = Geo<N, LR>;
-    const int n = a.n, nn = n * n;
-#pragma unroll
-    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
-        const int kk = e / G::NN, idx = e % G::NN;
-        const int r = idx / N, c = idx % N;
-        const long k = blk * G::KPW + kk;
-        if (r < n && c < n && k < a.nk) {
-            const cd v = lds[kk * Lay::MS + Lay::at(r, c)];
-            reinterpret_cast<double2 *>(a.ucache)[k * nn + r * n + c] = make_double2(v.x, v.y);
-        }
-    }
-}
-
-// Error norm of the attempt over this block's k-points (last stage only): the
-// stage input's x, k1, k3..k6 are read again (HBM that the VALU-bound kernel
-// leaves idle), k7 = d rho/dt comes from LDS natural layout; same lane ->
-// entry mapping as load_rho, two entries' loads in flight at a time.
-template <int N, int LR, class Lay = SwLay<N>>
-__device__ __forceinline__ double stage_err(const StageArgs &a, const cd *lds, long blk)
-{
-    using G = Geo<N, LR>;
-    const int n = a.n;
-    const long k0 = blk * G::KPW;
-    const double2 *x2 = reinterpret_cast<const double2 *>(a.in.x);
-    double v = 0;
-    constexpr int IT = G::KPW * G::NN / 64;
-#pragma unroll 2
-    for (int u = 0; u < IT; u++) {
-        const int e = threadIdx.x + 64 * u;
-        const int kk = e / G::NN, idx = e % G::NN, r = idx / N, c = idx % N;
-        const long k = k0 + kk;
-        if (r < n && c < n && k < a.nk) {
-            const long i = 8 + k * a.P + r * n + c;
-            double2 kv[6];
-#pragma unroll
-            for (int j = 0; j < 5; j++) kv[j] = reinterpret_cast<const double2 *>(a.in.k[j])[i];
-            const cd d = lds[kk * Lay::MS + Lay::at(r, c)];
-            kv[5] = make_double2(d.x, d.y);
-            v = fmax(v, err_entry(x2[i], kv, a.err_e, a.err_ea, a.err_er, a.err_dt));
-        }
-    }
-    if (blk == 0 && threadIdx.x < 8) {
-        // special slots: d/dt of A_t is -E_t (t < dim), the rest 0
-        const int t = threadIdx.x;
-        double2 kv[6];
-#pragma unroll
-        for (int j = 0; j < 5; j++) kv[j] = reinterpret_cast<const double2 *>(a.in.k[j])[t];
-        kv[5] = make_double2(t < a.dim ? -a.E[t] : 0.0, 0.0);
-        v = fmax(v, err_entry(x2[t], kv, a.err_e, a.err_ea, a.err_er, a.err_dt));
-    }
-    return v;
-}
-
-// special slots of the stage: d A_t/dt = -E_t (t < dim), and the stage input
-// itself where it is stored (block 0)
-__device__ __forceinline__ void stage_specials(const StageArgs &a, const long blk)
-{
-    if (blk == 0 && threadIdx.x < 8) {
-        const int t = threadIdx.x;
-        double v = t < a.dim ? -a.E[t] : 0.0;
-        a.dxdt[2 * t] = v;
-        a.dxdt[2 * t + 1] = 0.0;
-        if (a.xout) {
-            double re = a.in.x[2 * t], im = a.in.x[2 * t + 1];
-#pragma unroll
-            for (int j = 0; j < 6; j++)
-                if (j < a.in.nt) {
-                    re += a.in.c[j] * a.in.k[j][2 * t];
-                    im += a.in.c[j] * a.in.k[j][2 * t + 1];
-                }
-            a.xout[2 * t] = re;
-            a.xout[2 * t + 1] = im;
-        }
-    }
-}
-
-// The stage for 16-band layouts: the Jacobi in the XOR-wrapped register
-// layout, every matrix product on the matrix cores (mfma_cprod), one k-point
-// of the block at a time with all 64 lanes; products hand over through LDS
-// in natural layout.  Same operations as the generic body below.
-template <int LR>
-__device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long blk)
-{
-    constexpr int N = 16;
-    using G = Geo<N, LR>;
-    constexpr int C = G::C, KPW = G::KPW;
-    __shared__ cd S0[KPW * G::NN];
-    __shared__ cd S1[KPW * G::NN];
-    __shared__ cd S2[KPW * G::NN];
-    __shared__ double Dg[KPW * N];
-    __shared__ cd Ja[KPW * N], Jb[KPW * N];
-    __shared__ int Fa[KPW * N];
-
-    const Lane L = lane_id<N, LR>(a.nk, blk);
-    const int n = a.n, nn = n * n;
-    cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
-    stage_specials(a, blk);
-
-    if (a.dephase) {
-        cd A[C], U[C];
-        if (a.warm) {
-            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
-            load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
-            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
-            wsync();
-            load_u<N, LR>(s0, L, U);
-            u_pad_identity<N, LR>(L, n, U);
-            CAcc T[KPW];
-#pragma unroll
-            for (int kk = 0; kk < KPW; kk++) {
-                cacc_zero(T[kk]);
-                mfma_cprod<0, 0>(S0 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // U' H0
-            }
-            wsync();
-#pragma unroll
-            for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, T[kk]);
-            store_u<N, LR>(s1, L, U);
-            wsync();
-#pragma unroll
-            for (int kk = 0; kk < KPW; kk++) {
-                cacc_zero(T[kk]);
-                mfma_cprod<0, 1>(S0 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // (U' H0) U'^dagger
-            }
-            wsync();
-#pragma unroll
-            for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, T[kk]);
-            wsync();
-            load_xrow<N, LR>(s0, L, A);
-            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
-        } else {
-            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
-            wsync();
-            load_xrow<N, LR>(s1, L, A);
-            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
-        }
-        store_u<N, LR>(s1, L, U);  // U -> S1
-        wsync();
-        store_ucache<N, LR>(a, S1, blk);
-    }
-    load_rho<N, LR>(S0, a, blk);  // rho -> S0 (S1 still holds U)
-    wsync();
-    if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
-    // the dephasing first, on the side buffer S2 (rho stays in S0), then the
-    // coherent part, accumulated onto its last product: the result lives in
-    // registers only from there to the store
-    CAcc D[KPW];
-#pragma unroll
-    for (int kk = 0; kk < KPW; kk++) cacc_zero(D[kk]);
-    if (a.dephase) {
-        // U^dagger (D o (U rho U^dagger)) U
-        CAcc T[KPW];
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) {
-            cacc_zero(T[kk]);
-            mfma_cprod<0, 0>(S1 + kk * G::NN, S0 + kk * G::NN, T[kk]);  // T = U rho
-            cacc_store(S2 + kk * G::NN, T[kk]);
-        }
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) {
-            cacc_zero(T[kk]);
-            mfma_cprod<0, 1>(S2 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // rhoH = T U^dagger
-            const int b = cacc_col();
-            const double eb = Dg[kk * N + b];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int ar = cacc_row(j);
-                double gam;
-                if (a.w < 0) {
-                    gam = (ar != b) ? 1.0 : 0.0;
-                } else {
-                    const double dE = (Dg[kk * N + ar] - eb) / a.w;
-                    gam = 1 - exp(-dE * dE);
-                }
-                const double f = -gam * a.F2;
-                T[kk].r[j] *= f;
-                T[kk].i[j] *= f;
-            }
-        }
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) cacc_store(S2 + kk * G::NN, T[kk]);
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) {
-            cacc_zero(T[kk]);
-            mfma_cprod<0, 0>(S2 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // S = rhoH' U
-        }
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) cacc_store(S2 + kk * G::NN, T[kk]);
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) mfma_cprod<1, 0>(S1 + kk * G::NN, S2 + kk * G::NN, D[kk]);  // U^dagger S
-        wsync();
-    }
-    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
-    wsync();
-    // coherent part -i (H_E rho - rho H_E): D + (-i) X = D + (X.i, -X.r), so
-    // it is accumulated as i (rho H_E - H_E rho) with the roles of the real and
-    // imaginary accumulators exchanged
-#pragma unroll
-    for (int kk = 0; kk < KPW; kk++) {
-        CAcc t;
-        t.r = D[kk].i;
-        t.i = -D[kk].r;
-        mfma_cprod<0, 0, -1>(S1 + kk * G::NN, S0 + kk * G::NN, t);  // - H_E rho
-        mfma_cprod<0, 0>(S0 + kk * G::NN, S1 + kk * G::NN, t);      // + rho H_E
-        // t = i D + (rho H_E - H_E rho) = i (D - i (H_E rho - rho H_E))
-        D[kk].r = -t.i;
-        D[kk].i = t.r;
-    }
-    // d rho/dt (with zero pads) through LDS for coalescing
-    wsync();
-#pragma unroll
-    for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, D[kk]);
-    wsync();
-    store_state<N, LR>(a.dxdt, S0, n, a.P, a.nk, blk);
-    if (a.err_partial) {
-        const double v = seg_max<64>(stage_err<N, LR>(a, S0, blk));
-        if (threadIdx.x == 0) a.err_partial[blk] = v;
-    }
-}
-
-// ---- one lane per matrix row (LR = 1, N <= 8): one LDS matrix per k-point --
-// Lane i of a k-point holds row i of a matrix in registers, register c the
-// entry (i, i ^ c) ("rows"), or column i, register c the entry (i ^ c, i)
-// ("columns").  A product takes its left operand from registers and its right
-// operand from the k-point's LDS matrix in SlotLay, so the stage needs one
-// LDS matrix per k-point instead of two (three waves per SIMD fit in LDS).
-template <int N>
-__device__ __forceinline__ void slot_store_rows(cd *s, int i, const cd *v)
-{
-#pragma unroll
-    for (int c = 0; c < N; c++) s[i * SlotLay<N>::RS + c] = v[c];
-}
-template <int N>
-__device__ __forceinline__ void slot_load_rows(const cd *s, int i, cd *v)
-{
-#pragma unroll
-    for (int c = 0; c < N; c++) v[c] = s[i * SlotLay<N>::RS + c];
-}
-template <int N>
-__device__ __forceinline__ void slot_store_cols(cd *s, int i, const cd *v)
-{
-#pragma unroll
-    for (int c = 0; c < N; c++) s[(i ^ c) * SlotLay<N>::RS + c] = v[c];
-}
-template <int N>
-__device__ __forceinline__ void slot_load_cols(const cd *s, int i, cd *v)
-{
-#pragma unroll
-    for (int c = 0; c < N; c++) v[c] = s[(i ^ c) * SlotLay<N>::RS + c];
-}
-
-// out[c] = sum_l X[i][l] Y[l][i ^ c], l = i ^ c'.  X from registers: XOP 0,
-// xr = rows of X; XOP 1, xr = columns of V and X = V^dagger.  Y from LDS
-// (SlotLay): YOP 0, Y = Ys; YOP 1, Y = Ys^dagger.  One register's loads are
-// in flight at a time (the scheduler may not hoist the next ones), which
-// keeps the product's footprint at one row of operands.
-template <int N, int XOP, int YOP>
-__device__ __forceinline__ void reg_product(const cd *xr, const cd *s, int i, cd *out)
-{
-    constexpr int RS = SlotLay<N>::RS;
-#pragma unroll
-    for (int c = 0; c < N; c++) out[c] = mk(0, 0);
-#pragma unroll
-    for (int cp = 0; cp < N; cp++) {
-        const cd x = XOP ? cconj(xr[cp]) : xr[cp];
-        if constexpr (YOP == 0) {
-            // Y[l][j], l = i ^ cp, j = i ^ c: row l, slot cp ^ c
-            const cd *row = s + (i ^ cp) * RS;
-#pragma unroll
-            for (int c = 0; c < N; c++) out[c] = cfma(x, row[cp ^ c], out[c]);
-        } else {
-            // conj(Ys[j][l]): row j = i ^ c, slot c ^ cp
-#pragma unroll
-            for (int c = 0; c < N; c++) out[c] = cfma_jc(x, s[(i ^ c) * RS + (c ^ cp)], out[c]);
-        }
-        pin_rows<N>(out);
-    }
-}
-
-template <int N>
-__device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long blk)
-{
-    using G = Geo<N, 1>;
-    using Lay = SlotLay<N>;
-    constexpr int C = N, KPW = G::KPW, RS = Lay::RS;
-    __shared__ cd S[KPW * Lay::MS];
-    __shared__ double Dg[KPW * N];
-    __shared__ cd Ja[KPW * N], Jb[KPW * N];
-    __shared__ int Fa[KPW * N];
-
-    const Lane L = lane_id<N, 1>(a.nk, blk);
-    const int n = a.n, nn = n * n;
-    cd *s = S + L.kk * Lay::MS;
-    stage_specials(a, blk);
-
-    cd U[C];   // eigenvectors, columns
-    cd T1[C];  // U rho, rows
-    if (a.dephase) {
-        cd A[C];
-        if (a.warm) {
-            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
-            load_mats<N, 1, Lay>(S, a.ucache, nn, 0, n, a.nk, blk);  // U' -> S
-            wsync();
-            cd T[C];
-            slot_load_rows<N>(s, L.i, T);
-            slot_load_cols<N>(s, L.i, U);
-            // padded rows and columns of U' are the identity
-            u_pad_identity<N, 1>(L, n, T);
-            u_pad_identity<N, 1>(L, n, U);
-            wsync();
-            phase_mats<N, 1, Lay>(S, a, 0, blk);  // H0 -> S
-            wsync();
-            reg_product<N, 0, 0>(T, s, L.i, A);  // U' H0
-            wsync();
-            slot_store_cols<N>(s, L.i, U);  // U' -> S
-            wsync();
-#pragma unroll
-            for (int c = 0; c < C; c++) T[c] = A[c];
-            reg_product<N, 0, 1>(T, s, L.i, A);  // (U' H0) U'^dagger
-            wave_jacobi<N, 1>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
-        } else {
-            phase_mats<N, 1, Lay>(S, a, 0, blk);  // H0 -> S
-            wsync();
-            slot_load_rows<N>(s, L.i, A);
-            wave_jacobi<N, 1>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
-        }
-        wsync();
-        slot_store_cols<N>(s, L.i, U);  // U -> S
-        wsync();
-        store_ucache<N, 1, Lay>(a, S, blk);
-        slot_load_rows<N>(s, L.i, A);  // U rows
-        wsync();
-        load_rho<N, 1, Lay>(S, a, blk);  // rho -> S
-        wsync();
-        reg_product<N, 0, 0>(A, s, L.i, T1);  // U rho
-    } else {
-        load_rho<N, 1, Lay>(S, a, blk);  // rho -> S
-        wsync();
-    }
-    if (a.xout) store_state<N, 1, Lay>(a.xout, S, n, a.P, a.nk, blk);
-    cd dr[C];
-    {
-        // coherent part -i (H_E rho - rho H_E), with H_E rho = (rho H_E)^dagger
-        cd rr[C], T[C];
-        slot_load_rows<N>(s, L.i, rr);
-        wsync();
-        phase_mats<N, 1, Lay>(S, a, 1, blk);  // H_E -> S
-        wsync();
-        reg_product<N, 0, 0>(rr, s, L.i, T);  // rho H_E
-        wsync();
-        slot_store_rows<N>(s, L.i, T);
-        wsync();
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            const cd th = cconj(s[(L.i ^ c) * RS + c]);
-            const cd d = csub(th, T[c]);
-            dr[c] = mk(d.y, -d.x);
-        }
-    }
-    if (a.dephase) {
-        // dephasing in the eigenbasis: U^dagger (D o (U rho U^dagger)) U
-        cd T[C];
-        wsync();
-        slot_store_cols<N>(s, L.i, U);  // U -> S
-        wsync();
-        reg_product<N, 0, 1>(T1, s, L.i, T);  // rhoH = (U rho) U^dagger
-        const double di = Dg[L.kk * N + L.i];
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            const int j = L.i ^ c;
-            double gam;
-            if (a.w < 0) {
-                gam = (L.i != j) ? 1.0 : 0.0;
-            } else {
-                const double dE = (di - Dg[L.kk * N + j]) / a.w;
-                gam = 1 - exp(-dE * dE);
-            }
-            T[c] = cscale(T[c], -gam * a.F2);
-        }
-        reg_product<N, 0, 0>(T, s, L.i, T1);  // Q = rhoH' U
-        wsync();
-        slot_store_rows<N>(s, L.i, T1);  // Q -> S
-        wsync();
-        reg_product<N, 1, 0>(U, s, L.i, T);  // U^dagger Q
-#pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = cadd(dr[c], T[c]);
-    }
-    // d rho/dt through LDS for coalescing
-    wsync();
-    slot_store_rows<N>(s, L.i, dr);
-    wsync();
-    store_state<N, 1, Lay>(a.dxdt, S, n, a.P, a.nk, blk);
-    if (a.err_partial) {
-        const double v = seg_max<64>(stage_err<N, 1, Lay>(a, S, blk));
-        if (threadIdx.x == 0) a.err_partial[blk] = v;
-    }
-}
-
-// the stage for the KPW k-points of logical block blk of one instance
-template <int N, int LR>
-__device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
-{
-    using G = Geo<N, LR>;
-    constexpr int C = G::C;
-    if constexpr (N == 16) {
-        rhs_stage_body16<LR>(a, blk);
-        return;
-    } else if constexpr (LR == 1) {
-        rhs_stage_body_r1<N>(a, blk);
-        return;
-    } else {
-    __shared__ cd S0[G::KPW * G::NN];
-    __shared__ cd S1[G::KPW * G::NN];
-    __shared__ double Dg[G::KPW * N];
-    __shared__ cd Ja[G::KPW * N], Jb[G::KPW * N];
-    __shared__ int Fa[G::KPW * N];
-
-    const Lane L = lane_id<N, LR>(a.nk, blk);
-    const int n = a.n, nn = n * n;
-    cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
-
-    if (blk == 0 && threadIdx.x < 8) {
-        const int t = threadIdx.x;
-        double v = t < a.dim ? -a.E[t] : 0.0;
-        a.dxdt[2 * t] = v;
-        a.dxdt[2 * t + 1] = 0.0;
-        if (a.xout) {
-            double re = a.in.x[2 * t], im = a.in.x[2 * t + 1];
-#pragma unroll
-            for (int j = 0; j < 6; j++)
-                if (j < a.in.nt) {
-                    re += a.in.c[j] * a.in.k[j][2 * t];
-                    im += a.in.c[j] * a.in.k[j][2 * t + 1];
-                }
-            a.xout[2 * t] = re;
-            a.xout[2 * t + 1] = im;
-        }
-    }
-
-    cd dr[C];
-    if (a.dephase) {
-        cd A[C], U[C];
-        if (a.warm) {
-            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
-            load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
-            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
-            wsync();
-            load_u<N, LR>(s0, L, U);
-            // padded rows and columns of U' are the identity
-            u_pad_identity<N, LR>(L, n, U);
-            cd T[C];
-            lds_product<N, LR, 0, 0>(s0, s1, L, T);  // U' H0
-            wsync();
-            store_xrow<N, LR>(s0, L, T);
-            store_u<N, LR>(s1, L, U);
-            wsync();
-            lds_product<N, LR, 0, 1>(s0, s1, L, A);  // (U' H0) U'^dagger
-            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
-        } else {
-            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
-            wsync();
-            load_xrow<N, LR>(s1, L, A);
-            wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
-        }
-        store_u<N, LR>(s1, L, U);  // U -> S1
-        wsync();
-        // the eigenvectors go to the per-k cache: warm start of the next call,
-        // and the copy the dephasing below reloads (each lane reads back only
-        // the entries it stored itself, same-address order within a thread)
-        store_ucache<N, LR>(a, S1, blk);
-    }
-    load_rho<N, LR>(S0, a, blk);  // rho -> S0
-    wsync();
-    if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
-    // coherent part: -i (H_E rho - rho H_E)
-    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
-    wsync();
-    {
-        cd t1[C];
-        lds_product<N, LR, 0, 0>(s1, s0, L, t1);
-#pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = mk(t1[c].y, -t1[c].x);
-        lds_product<N, LR, 0, 0>(s0, s1, L, t1);
-#pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = mk(dr[c].x - t1[c].y, dr[c].y + t1[c].x);
-    }
-    if (a.dephase) {
-        // dephasing in the eigenbasis: U^dagger (D o (U rho U^dagger)) U
-        wsync();
-        load_mats<N, LR>(S1, a.ucache, nn, 0, n, a.nk, blk);  // U -> S1
-        wsync();
-        cd T[C];
-        lds_product<N, LR, 0, 0>(s1, s0, L, T);  // T = U rho
-        wsync();
-        store_xrow<N, LR>(s0, L, T);
-        wsync();
-        lds_product<N, LR, 0, 1>(s0, s1, L, T);  // rhoH = T U^dagger
-        const double di = Dg[L.kk * N + L.i];
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            const int j = L.i ^ (L.h * C + c);
-            double gam;
-            if (a.w < 0) {
-                gam = (L.i != j) ? 1.0 : 0.0;
-            } else {
-                double dE = (di - Dg[L.kk * N + j]) / a.w;
-                gam = 1 - exp(-dE * dE);
-            }
-            T[c] = cscale(T[c], -gam * a.F2);
-        }
-        wsync();
-        store_xrow<N, LR>(s0, L, T);
-        wsync();
-        lds_product<N, LR, 0, 0>(s0, s1, L, T);  // S = rhoH' U
-        wsync();
-        store_xrow<N, LR>(s0, L, T);
-        wsync();
-        lds_product<N, LR, 1, 0>(s1, s0, L, T);  // U^dagger S
-#pragma unroll
-        for (int c = 0; c < C; c++) dr[c] = cadd(dr[c], T[c]);
-    }
-
-    // write d rho/dt (with zero pads) through LDS for coalescing
-    wsync();
-    store_xrow<N, LR>(s0, L, dr);
-    wsync();
-    store_state<N, LR>(a.dxdt, S0, n, a.P, a.nk, blk);
-    if (a.err_partial) {
-        const double v = seg_max<64>(stage_err<N, LR>(a, S0, blk));
-        if (threadIdx.x == 0) a.err_partial[blk] = v;
-    }
-    }
-}
-
-// minimum waves per SIMD the register allocation must allow: the 16-band
-// kernels fit three (138 VGPRs at 4 lanes per row) when the allocator is told so, instead of
-// two with part of the matrix-core operands in AGPRs
-template <int N>
-constexpr int stage_min_waves() { return N == 16 ? 3 : 1; }
-
-template <int N, int LR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N>())))
-void rhs_stage_kernel(StageArgs a)
-{
-    rhs_stage_body<N, LR>(a, xcd_block(blockIdx.x, gridDim.x));
-}
-
-// Several instances in one launch: entry j owns the logical blocks
-// [block0[j], block0[j+1]); its arguments are read once into registers.
-template <int N, int LR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N>())))
-void rhs_stage_batched_kernel(const StageArgs *args, const long *block0, int m)
-{
-    const long gb = xcd_block(blockIdx.x, gridDim.x);
-    const int j = find_entry(block0, m, gb);
-    // read in place (a local copy of the 256-byte struct would live in scratch)
-    rhs_stage_body<N, LR>(args[j], gb - block0[j]);
-}
-
-// initial Fermi-Dirac state rho = U^dagger f(eps) U, and/or eigenvalues and
-// eigenvectors (code/src/PropagatorCPU.cpp:31-70, code/src/main/runSBE.cpp:130-155)
-template <int N, int LR>
-__global__ __launch_bounds__(64) void init_state_kernel(InitArgs a)
-{
-    using G = Geo<N, LR>;
-    constexpr int C = G::C;
-    __shared__ cd S0[G::KPW * G::NN];
-    __shared__ cd S1[G::KPW * G::NN];
-    __shared__ double Dg[G::KPW * N];
-    __shared__ cd Ja[G::KPW * N], Jb[G::KPW * N];
-    __shared__ int Fa[G::KPW * N];
-    const long blk = blockIdx.x;
-    const Lane L = lane_id<N, LR>(a.nk, blk);
-    const int n = a.n, nn = n * n;
-    cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
-    load_mats<N, LR>(S1, a.mesh, (long)a.ops * nn, 0, n, a.nk, blk);
-    wsync();
-    cd A[C], U[C];
-    load_xrow<N, LR>(s1, L, A);
-    wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
-    wsync();
-    u_rows<N, LR>(s1, L, U);
-    if (a.eig && L.valid && L.h == 0 && L.i < n) a.eig[L.k * n + L.i] = Dg[L.kk * N + L.i];
-    if (a.U && L.valid && L.i < n) {
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            int j = L.i ^ (L.h * C + c);
-            if (j < n) {
-                double *o = a.U + 2 * ((L.k * n + L.i) * n + j);
-                o[0] = U[c].x;
-                o[1] = U[c].y;
-            }
-        }
-    }
-    if (!a.state) return;
-    // U -> S1, f(eps) U -> S0
-    store_xrow<N, LR>(s1, L, U);
-    const double f = 1.0 / (1.0 + exp((Dg[L.kk * N + L.i] - a.mu) / a.kT));
-    cd FU[C];
-#pragma unroll
-    for (int c = 0; c < C; c++) FU[c] = (L.i < n) ? cscale(U[c], f) : mk(0, 0);
-    store_xrow<N, LR>(s0, L, FU);
-    wsync();
-    // rho[a][b] = sum_i conj(U[i][a]) f_i U[i][b]
-    cd R[C];
-    lds_product<N, LR, 1, 0>(s1, s0, L, R);
-    wsync();
-    store_xrow<N, LR>(s0, L, R);
-    wsync();
-    if (blockIdx.x == 0 && threadIdx.x < 8) {
-        a.state[2 * threadIdx.x] = 0;
-        a.state[2 * threadIdx.x + 1] = 0;
-    }
-    const long k0 = (long)blockIdx.x * G::KPW;
-    const int P = a.P;
-    for (int e = threadIdx.x; e < G::KPW * P; e += 64) {
-        int kk = e / P, p = e % P;
-        long k = k0 + kk;
-        if (k >= a.nk) break;
-        cd v = mk(0, 0);
-        if (p < nn) v = S0[kk * G::NN + sw<N>(p / n, p % n)];
-        double *o = a.state + 2 * (8 + k * P + p);
-        o[0] = v.x;
-        o[1] = v.y;
-    }
-}
-
-// H0(k + A) of the observer's partial phase sum (op 0 of M-wide rows) for the
-// wave's k-points into LDS natural layout (last-axis contraction per entry)
-template <int N, int LR>
-__device__ __forceinline__ void obs_h0_mats(cd *lds, const OccArgs &a, long blk)
-{
-    using G = Geo<N, LR>;
-    const int n = a.n;
-    for (int e = threadIdx.x; e < G::KPW * G::NN; e += 64) {
-        const int kk = e / G::NN, idx = e % G::NN;
-        const int r = idx / N, c = idx % N;
-        const long k = blk * G::KPW + kk;
-        cd v = mk(0, 0);
-        if (r < n && c < n && k < a.nk) v = phase_last(a.ph, k, a.M, r * n + c);
-        lds[kk * G::NN + sw<N>(r, c)] = v;
-    }
-}
-
-// Hamiltonian and Wannier occupations per k-point (ObserverCPU::operator(),
-// code/src/ObserverCPU.cpp:211-245): U H0(k + A) U^dagger = diag(e) by the
-// wavefront Jacobi (from the identity), occupations (U rho U^dagger)_bb,
-// bands in ascending energy (ties by index), smeared over energy with
-// weights exp(-((e_s - e_q)/smear)^2) summed over q in ascending order.
-template <int N, int LR>
-__global__ __launch_bounds__(64) void occupation_kernel(OccArgs a)
-{
-    using G = Geo<N, LR>;
-    constexpr int C = G::C;
-    __shared__ cd S0[G::KPW * G::NN];
-    __shared__ cd S1[G::KPW * G::NN];
-    __shared__ double Dg[G::KPW * N];
-    __shared__ cd Ja[G::KPW * N], Jb[G::KPW * N];
-    __shared__ int Fa[G::KPW * N];
-    const long blk = blockIdx.x;
-    const Lane L = lane_id<N, LR>(a.nk, blk);
-    const int n = a.n;
-    cd *s0 = S0 + L.kk * G::NN, *s1 = S1 + L.kk * G::NN;
-    obs_h0_mats<N, LR>(S1, a, blk);
-    load_mats<N, LR>(S0, a.x, a.P, 8, n, a.nk, blk);
-    wsync();
-    cd A[C], U[C];
-    load_xrow<N, LR>(s1, L, A);
-    wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
-    wsync();
-    u_rows<N, LR>(s1, L, U);
-    store_xrow<N, LR>(s1, L, U);
-    wsync();
-    cd T[C];
-    lds_product<N, LR, 0, 0>(s1, s0, L, T);  // T = U rho
-    // (U rho U^dagger)_ii = sum_j T_ij conj(U_ij) over the row's lanes
-    double o = 0;
-#pragma unroll
-    for (int c = 0; c < C; c++) o += T[c].x * U[c].x + T[c].y * U[c].y;
-    o = seg_sum<LR>(o);
-    const double ei = Dg[L.kk * N + L.i];
-    int rank = 0;
-    for (int j = 0; j < n; j++) {
-        const double ej = Dg[L.kk * N + j];
-        rank += (ej < ei) || (ej == ei && j < L.i);
-    }
-    wsync();
-    if (L.h == 0 && L.i < n) Ja[L.kk * N + rank] = mk(ei, o);
-    wsync();
-    if (L.h == 0 && L.i < n && L.valid) {
-        const int sidx = L.i;  // the band this lane reports (ascending energy)
-        const cd es = Ja[L.kk * N + sidx];
-        double v = es.y;
-        if (a.smear > 0) {
-            double ws = 0, acc = 0;
-            for (int q = 0; q < n; q++) {
-                const cd eq = Ja[L.kk * N + q];
-                const double dE = (es.x - eq.x) / a.smear;
-                const double w = exp(-dE * dE);
-                ws += w;
-                acc += eq.y * w;
-            }
-            v = acc / ws;
-        }
-        double *row = a.X + L.k * (long)a.Q;
-        row[3 + sidx] = v;
-        row[3 + n + sidx] = s0[sw<N>(sidx, sidx)].x;
-    }
-}
 
 template <int N, int LR>
 static hipError_t launch_occ_NL(const OccArgs &a, hipStream_t s)
@@ -1370,41 +22,6 @@ static hipError_t launch_rhs_NL(const StageArgs &a, hipStream_t s)
     if (blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL((rhs_stage_kernel<N, LR>), dim3((unsigned)blocks), dim3(64), 0, s, a);
     return hipGetLastError();
-}
-
-template <int N, int LR>
-__global__ __launch_bounds__(64) void rhs_stage_set_kernel(
-    StageSet ss)
-{
-    const long blk = xcd_block(blockIdx.x, gridDim.x);
-    for (int q = 0; q < ss.m; q++) {
-        rhs_stage_body<N, LR>(ss.st[q], blk);
-        // the next stage loads, from other lanes, what this one stored
-        __threadfence_block();
-        wsync();
-    }
-    if (ss.ticket) {
-        // thread 0 stored this block's error maximum; its ticket releases it.
-        // The block that takes the last ticket reduces all of them, resets the
-        // counter for the next launch and stores the norm's bit pattern with a
-        // system-scope store (no block waits for another)
-        unsigned t = 0;
-        if (threadIdx.x == 0) t = __hip_atomic_fetch_add(ss.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        t = __shfl(t, 0);
-        if (t + 1 == gridDim.x) {
-            __threadfence();
-            const double *part = ss.st[ss.m - 1].err_partial;
-            double v = 0;
-            for (unsigned i = threadIdx.x; i < gridDim.x; i += 64)
-                v = fmax(v, __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            v = seg_max<64>(v);
-            if (threadIdx.x == 0) {
-                __hip_atomic_store(ss.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(ss.err_out, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
-            }
-        }
-    }
 }
 
 template <int N, int LR>

@@ -138,7 +138,8 @@ class Inputs:
         self.params.eps_rel = cfg.eps_rel
         self.params.sweep_count = cfg.sweeps
         self.params.flags = (1 if getattr(cfg, "cold_jacobi", False) else 0) | \
-            (2 if getattr(cfg, "gpu_layout", False) else 0)
+            (2 if getattr(cfg, "gpu_layout", False) else 0) | \
+            (4 if getattr(cfg, "device_controller", False) else 0)
         self.pulses = pulse_array(cfg.pulses)
         self.num_pulses = len(cfg.pulses)
 

@@ -122,6 +122,7 @@ class Config:
     k_global_shift: tuple = (0.0, 0.0, 0.0)
     cold_jacobi: bool = False   # start each RHS's Jacobi from the identity
     gpu_layout: bool = False    # state vectors in the reference's GPU layout (rho_k unpadded)
+    device_controller: bool = False  # 1-D, n <= 8: the whole integration in one launch (step control on the GPU)
 
     @property
     def num_k(self) -> int:

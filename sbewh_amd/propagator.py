@@ -113,21 +113,27 @@ class SBEContext:
     def integrate(self, state: np.ndarray, output_count: int, replay: StepLog | None = None,
                   log_capacity: int = 400000, observer=None) -> dict:
         """device-resident integrate_const; `state` is advanced in place;
-        observer: an optional FullObserver recording every output time."""
+        observer: an optional FullObserver recording every output time;
+        log_capacity 0: no step log (with Config.device_controller, 1-D and
+        n <= 8: the resident integrator, step control on the GPU)."""
         assert state.dtype == np.complex128 and state.flags.c_contiguous
         t = np.zeros(output_count)
         j = np.zeros((output_count, 3))
-        log = StepLog(log_capacity)
+        log = StepLog(log_capacity) if log_capacity else None
+        lp = C.byref(log.c) if log is not None else None
         rhs = C.c_long()
         rp = C.byref(replay.c) if replay is not None else None
         if observer is not None:
             steps = load().sbe_integrate_observed(self.h, dptr(state.view(float)), output_count, observer.h, dptr(t),
-                                                  dptr(j.reshape(-1)), C.byref(log.c), rp, C.byref(rhs))
+                                                  dptr(j.reshape(-1)), lp, rp, C.byref(rhs))
         else:
             steps = load().sbe_integrate(self.h, dptr(state.view(float)), output_count, dptr(t),
-                                         dptr(j.reshape(-1)), C.byref(log.c), rp, C.byref(rhs))
+                                         dptr(j.reshape(-1)), lp, rp, C.byref(rhs))
         check(steps, self.h)
-        return {"t": t, "j": j, "steps": int(steps), "rhs_calls": int(rhs.value), "log": log}
+        dev = C.c_int()
+        check(load().sbe_integrate_info(self.h, C.byref(dev)), self.h)
+        return {"t": t, "j": j, "steps": int(steps), "rhs_calls": int(rhs.value), "log": log,
+                "device_controller": bool(dev.value)}
 
     def hk_mesh(self, A=(0, 0, 0), E=(0, 0, 0)) -> np.ndarray:
         out = np.zeros((self.num_k, 2, self.n, self.n), complex)

@@ -62,7 +62,7 @@ def test_invalid_configurations_are_rejected():
     assert rc == -1 and "R=(0,0,0)" in msg
     rc, msg, _ = _create(cfg, lambda i: setattr(i, "num_pulses", 0))
     assert rc == -1
-    rc, msg, _ = _create(cfg, lambda i: setattr(i.params, "flags", 4))
+    rc, msg, _ = _create(cfg, lambda i: setattr(i.params, "flags", 8))
     assert rc == -1 and "flags" in msg
     L = _lib.load()
     assert L.sbe_jacobi_batch(0, 5, 1, None, None, None) == -1
