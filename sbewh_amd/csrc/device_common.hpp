@@ -79,6 +79,17 @@ __device__ __forceinline__ int find_entry(const long *b0, int m, long gb)
     return lo;
 }
 
+// The same for m <= 64 with every lane of the wavefront active: one load per
+// lane and a ballot instead of log2(m) dependent loads (b0 ascending from 0,
+// so the entries with b0[j] <= gb are a prefix).
+__device__ __forceinline__ int find_entry_wave(const long *b0, int m, long gb)
+{
+    if (m > 64) return find_entry(b0, m, gb);
+    const int l = threadIdx.x & 63;
+    const bool le = l < m && b0[l] <= gb;
+    return __popcll(__ballot(le)) - 1;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD
 // b % 8).  This bijection gives each XCD one contiguous range of logical
 // blocks, so neighbouring k-points (which share phase-sum rows) meet in the

@@ -19,7 +19,7 @@ __global__ __launch_bounds__(64) void jk_kernel(ObsArgs a) { jk_body(a, blockIdx
 __global__ __launch_bounds__(64) void jk_batched_kernel(const ObsArgs *args, const long *block0, int m)
 {
     const long gb = blockIdx.x;
-    const int j = find_entry(block0, m, gb);
+    const int j = find_entry_wave(block0, m, gb);
     const ObsArgs a = args[j];
     jk_body(a, gb - block0[j]);
 }

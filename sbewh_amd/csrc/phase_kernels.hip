@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void contract_set_kernel(ContractSet cs)
 __global__ __launch_bounds__(256) void contract_batched_kernel(const ContractStage *st, const long *block0, int m)
 {
     const long gb = blockIdx.x;
-    const int j = find_entry(block0, m, gb);
+    const int j = find_entry_wave(block0, m, gb);
     const ContractStage q = st[j];
     stage_body(q, gb - block0[j]);
 }

@@ -1200,7 +1200,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_wa
 void rhs_stage_batched_kernel(const StageArgs *args, const long *block0, int m)
 {
     const long gb = xcd_block(blockIdx.x, gridDim.x);
-    const int j = find_entry(block0, m, gb);
+    const int j = find_entry_wave(block0, m, gb);
     // read in place (a local copy of the 256-byte struct would live in scratch)
     rhs_stage_body<N, LR>(args[j], gb - block0[j]);
 }
