@@ -327,11 +327,12 @@ __device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
 // (j, j ^ r) of U then combines register c with register c ^ (r mod C) of the
 // same lane (r < C) or of lane h ^ (r / C) in the row's own lane group, and it
 // uses the coefficients the column update of A reads anyway, instead of an
-// exchange with the partner row's lanes.  For 16 bands at 4 lanes per row
-// that trade would replace LDS permutes by more vector-ALU moves, so that
-// layout keeps U in rows.
+// exchange with the partner row's lanes.  Every layout uses it (for 16 bands
+// at 4 lanes per row it replaces the U half of the ds_bpermute exchanges of
+// rounds 4-15 by register selects: 2.14 ms per 48^3 launch against 2.20 with
+// U kept in rows).
 template <int N, int LR>
-constexpr bool jac_ut() { return !(N == 16 && LR == 4); }
+constexpr bool jac_ut() { return true; }
 
 // U registers (either layout) <-> LDS natural layout
 template <int N, int LR>
