@@ -400,21 +400,48 @@ __device__ __forceinline__ void cacc_zero(CAcc &c)
     c.i = d4{0, 0, 0, 0};
 }
 
-// acc += SX * X Y for one k-point's matrices in LDS natural layout
-// (XOP 1: X[i][l] := conj(Xs[l][i]); YOP 1: Y[l][j] := conj(Ys[j][l]));
-// every operand is read before the first matrix instruction
-template <int XOP, int YOP, int SX = 1>
-__device__ __forceinline__ void mfma_cprod(const cd *X, const cd *Y, CAcc &acc)
+// Operands of one complex product, four inner-index steps: left a[kc] =
+// X[l % 16][4 kc + l / 16], right b[kc] = Y[4 kc + l / 16][l % 16].
+// From one k-point's LDS matrix in natural layout (XOP 1: X[i][l] :=
+// conj(Xs[l][i]); YOP 1: Y[l][j] := conj(Ys[j][l])):
+template <int XOP>
+__device__ __forceinline__ void lds_left(const cd *X, cd *a)
 {
     const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-    cd a[4], b[4];
 #pragma unroll
     for (int kc = 0; kc < 4; kc++) {
         const int k = 4 * kc + q;
         a[kc] = (XOP == 0) ? X[sw<16>(r, k)] : cconj(X[sw<16>(k, r)]);
-        b[kc] = (YOP == 0) ? Y[sw<16>(k, r)] : cconj(Y[sw<16>(r, k)]);
-        if (SX < 0) a[kc] = mk(-a[kc].x, -a[kc].y);
     }
+}
+template <int YOP>
+__device__ __forceinline__ void lds_right(const cd *Y, cd *b)
+{
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+        const int k = 4 * kc + q;
+        b[kc] = (YOP == 0) ? Y[sw<16>(k, r)] : cconj(Y[sw<16>(r, k)]);
+    }
+}
+// From a finished accumulator M: accumulator register kc holds
+// M[4 kc + l / 16][l % 16], which is the right operand M itself, and, conjugated,
+// the left operand M^dagger; a product can so consume the previous one's
+// result without a trip through LDS.
+__device__ __forceinline__ void acc_right(const CAcc &m, cd *b)
+{
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) b[kc] = mk(m.r[kc], m.i[kc]);
+}
+__device__ __forceinline__ void acc_left_dagger(const CAcc &m, cd *a)
+{
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) a[kc] = mk(m.r[kc], -m.i[kc]);
+}
+
+// acc += X Y from operands a, b as above
+__device__ __forceinline__ void mfma_cmul(const cd *a, const cd *b, CAcc &acc)
+{
 #pragma unroll
     for (int kc = 0; kc < 4; kc++) {
         acc.r = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].x, b[kc].x, acc.r, 0, 0, 0);
@@ -422,6 +449,21 @@ __device__ __forceinline__ void mfma_cprod(const cd *X, const cd *Y, CAcc &acc)
         acc.r = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[kc].y, b[kc].y, acc.r, 0, 0, 0);
         acc.i = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc].y, b[kc].x, acc.i, 0, 0, 0);
     }
+}
+
+// acc += SX * X Y for one k-point's matrices in LDS natural layout; every
+// operand is read before the first matrix instruction
+template <int XOP, int YOP, int SX = 1>
+__device__ __forceinline__ void mfma_cprod(const cd *X, const cd *Y, CAcc &acc)
+{
+    cd a[4], b[4];
+    lds_left<XOP>(X, a);
+    lds_right<YOP>(Y, b);
+    if (SX < 0) {
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) a[kc] = mk(-a[kc].x, -a[kc].y);
+    }
+    mfma_cmul(a, b, acc);
 }
 
 // accumulator element j of this lane: row and column
@@ -720,9 +762,11 @@ __device__ __forceinline__ void stage_specials(const StageArgs &a, const long bl
 }
 
 // The stage for 16-band layouts: the Jacobi in the XOR-wrapped register
-// layout, every matrix product on the matrix cores (mfma_cprod), one k-point
-// of the block at a time with all 64 lanes; products hand over through LDS
-// in natural layout.  Same operations as the generic body below.
+// layout, every matrix product on the matrix cores, one k-point of the block
+// at a time with all 64 lanes.  A product's result stays in its accumulator
+// and feeds the next product as an operand (acc_right, acc_left_dagger), so
+// two LDS matrices per k-point suffice (8.9 KB for one k-point per block:
+// four wavefronts per SIMD fit).  Same operations as the generic body below.
 template <int LR>
 __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long blk)
 {
@@ -731,7 +775,6 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
     constexpr int C = G::C, KPW = G::KPW;
     __shared__ cd S0[KPW * G::NN];
     __shared__ cd S1[KPW * G::NN];
-    __shared__ cd S2[KPW * G::NN];
     __shared__ double Dg[KPW * N];
     __shared__ cd Ja[KPW * N], Jb[KPW * N];
     __shared__ int Fa[KPW * N];
@@ -744,7 +787,7 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
     if (a.dephase) {
         cd A[C], U[C];
         if (a.warm) {
-            // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
+            // warm start: A = U' (H0 U'^dagger) with U' the previous eigenvectors
             load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
             phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
             wsync();
@@ -753,18 +796,14 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
             CAcc T[KPW];
 #pragma unroll
             for (int kk = 0; kk < KPW; kk++) {
+                cd x[4], y[4];
+                CAcc t;
+                cacc_zero(t);
+                mfma_cprod<0, 1>(S1 + kk * G::NN, S0 + kk * G::NN, t);  // H0 U'^dagger
+                lds_left<0>(S0 + kk * G::NN, x);
+                acc_right(t, y);
                 cacc_zero(T[kk]);
-                mfma_cprod<0, 0>(S0 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // U' H0
-            }
-            wsync();
-#pragma unroll
-            for (int kk = 0; kk < KPW; kk++) cacc_store(S0 + kk * G::NN, T[kk]);
-            store_u<N, LR>(s1, L, U);
-            wsync();
-#pragma unroll
-            for (int kk = 0; kk < KPW; kk++) {
-                cacc_zero(T[kk]);
-                mfma_cprod<0, 1>(S0 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // (U' H0) U'^dagger
+                mfma_cmul(x, y, T[kk]);  // U' (H0 U'^dagger)
             }
             wsync();
 #pragma unroll
@@ -785,26 +824,26 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
     load_rho<N, LR>(S0, a, blk);  // rho -> S0 (S1 still holds U)
     wsync();
     if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
-    // the dephasing first, on the side buffer S2 (rho stays in S0), then the
-    // coherent part, accumulated onto its last product: the result lives in
-    // registers only from there to the store
+    // the dephasing first, from rho in S0 and U in S1 through the
+    // accumulators, then the coherent part, accumulated onto its result
     CAcc D[KPW];
 #pragma unroll
     for (int kk = 0; kk < KPW; kk++) cacc_zero(D[kk]);
     if (a.dephase) {
-        // U^dagger (D o (U rho U^dagger)) U
-        CAcc T[KPW];
+        // U^dagger (D o (U rho U^dagger))^dagger U: the bracket is Hermitian,
+        // and its conjugate transpose is what an accumulator hands over as a
+        // left operand
 #pragma unroll
         for (int kk = 0; kk < KPW; kk++) {
-            cacc_zero(T[kk]);
-            mfma_cprod<0, 0>(S1 + kk * G::NN, S0 + kk * G::NN, T[kk]);  // T = U rho
-            cacc_store(S2 + kk * G::NN, T[kk]);
-        }
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) {
-            cacc_zero(T[kk]);
-            mfma_cprod<0, 1>(S2 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // rhoH = T U^dagger
+            const cd *sr = S0 + kk * G::NN, *su = S1 + kk * G::NN;
+            cd x[4], y[4];
+            CAcc t, h;
+            cacc_zero(t);
+            mfma_cprod<0, 1>(sr, su, t);  // t = rho U^dagger
+            lds_left<0>(su, x);
+            acc_right(t, y);
+            cacc_zero(h);
+            mfma_cmul(x, y, h);  // rhoH = U t
             const int b = cacc_col();
             const double eb = Dg[kk * N + b];
 #pragma unroll
@@ -818,25 +857,17 @@ __device__ __forceinline__ void rhs_stage_body16(const StageArgs &a, const long 
                     gam = 1 - exp(-dE * dE);
                 }
                 const double f = -gam * a.F2;
-                T[kk].r[j] *= f;
-                T[kk].i[j] *= f;
+                h.r[j] *= f;
+                h.i[j] *= f;
             }
+            acc_left_dagger(h, x);
+            lds_right<0>(su, y);
+            cacc_zero(t);
+            mfma_cmul(x, y, t);  // S = rhoH'^dagger U
+            lds_left<1>(su, x);
+            acc_right(t, y);
+            mfma_cmul(x, y, D[kk]);  // U^dagger S
         }
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) cacc_store(S2 + kk * G::NN, T[kk]);
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) {
-            cacc_zero(T[kk]);
-            mfma_cprod<0, 0>(S2 + kk * G::NN, S1 + kk * G::NN, T[kk]);  // S = rhoH' U
-        }
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) cacc_store(S2 + kk * G::NN, T[kk]);
-        wsync();
-#pragma unroll
-        for (int kk = 0; kk < KPW; kk++) mfma_cprod<1, 0>(S1 + kk * G::NN, S2 + kk * G::NN, D[kk]);  // U^dagger S
         wsync();
     }
     phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
@@ -1182,13 +1213,15 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
 }
 
 // minimum waves per SIMD the register allocation must allow: the 16-band
-// kernels fit three (138 VGPRs at 4 lanes per row) when the allocator is told so, instead of
-// two with part of the matrix-core operands in AGPRs
-template <int N>
-constexpr int stage_min_waves() { return N == 16 ? 3 : 1; }
+// kernel at 4 lanes per row fits four (at most 128 VGPRs; 8.9 KB of LDS per
+// wavefront) when the allocator is told so, instead of two with part of the
+// matrix-core operands in AGPRs; at 2 lanes per row a bound would only buy
+// spills to scratch
+template <int N, int LR>
+constexpr int stage_min_waves() { return (N == 16 && LR == 4) ? 4 : 1; }
 
 template <int N, int LR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N>())))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N, LR>())))
 void rhs_stage_kernel(StageArgs a)
 {
     rhs_stage_body<N, LR>(a, xcd_block(blockIdx.x, gridDim.x));
@@ -1197,7 +1230,7 @@ void rhs_stage_kernel(StageArgs a)
 // Several instances in one launch: entry j owns the logical blocks
 // [block0[j], block0[j+1]); its arguments are read once into registers.
 template <int N, int LR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N>())))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<N, LR>())))
 void rhs_stage_batched_kernel(const StageArgs *args, const long *block0, int m)
 {
     const long gb = xcd_block(blockIdx.x, gridDim.x);

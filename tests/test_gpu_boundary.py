@@ -41,7 +41,7 @@ def to_gpu_layout(x_cpu, nk, n):
 
 
 @pytest.mark.parametrize("name", ["chain", "square"])
-def test_gpu_layout_rhs_equals_cpu_layout(name):
+def test_gpu_layout_rhs_equals_cpu_layout(name, monkeypatch):
     if name == "chain":
         cfg = cases.chain(sweeps=5)
         occ = 1
@@ -61,7 +61,11 @@ def test_gpu_layout_rhs_equals_cpu_layout(name):
     x = cases.perturbed_state(xc, n, cc.P, nk, seed=5)
     t = 0.3 * cc.time_window[0]
     dc = cc.rhs(x, t)
+    # bit-identical results need the same stage layout: each context times
+    # the layouts on its first RHS, so the second takes the first's choice
+    monkeypatch.setenv("SBEWH_STAGE_LR", str(cc.stage_tuning()["lanes_per_row"]))
     dg = cg.rhs(to_gpu_layout(x, nk, n), t)
+    assert cg.stage_tuning()["lanes_per_row"] == cc.stage_tuning()["lanes_per_row"]
     assert np.array_equal(to_gpu_layout(dc, nk, n), dg)
     assert np.array_equal(cc.observe(x), cg.observe(to_gpu_layout(x, nk, n)))
     o = Oracle(cfg)
@@ -108,12 +112,14 @@ def test_rhs_device_gpu_layout_on_device_buffers():
     cg.close()
 
 
-def test_integrate_gpu_layout_equals_cpu_layout():
+def test_integrate_gpu_layout_equals_cpu_layout(monkeypatch):
     cc, cg = SBEContext(chain_short(False)), SBEContext(chain_short(True))
     n, nk = cc.n, cc.num_k
     xc = cc.initial_state(cc.fermi_level(1))
     xg = to_gpu_layout(xc, nk, n)
     rc = cc.integrate(xc, 17)
+    # the same stage layout both ways (each context times them on its first RHS)
+    monkeypatch.setenv("SBEWH_STAGE_LR", str(cc.stage_tuning()["lanes_per_row"]))
     rg = cg.integrate(xg, 17)
     assert rc["steps"] == rg["steps"]
     assert np.array_equal(rc["j"], rg["j"])

@@ -212,6 +212,7 @@ def test_one_block_attempt_kernel(monkeypatch):
     st0 = o.initial_state(mu)
     so = st0.copy()
     ro = o.integrate(so, cfg.output_count)
+    monkeypatch.setenv("SBEWH_STAGE_LR", "1")   # the same layout both ways
     res = {}
     for fuse in ("512", "0"):
         monkeypatch.setenv("SBEWH_FUSE_BELOW", fuse)
