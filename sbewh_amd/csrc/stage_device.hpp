@@ -477,20 +477,6 @@ __device__ __forceinline__ void cacc_store(cd *lds, const CAcc &c)
     for (int j = 0; j < 4; j++) lds[sw<16>(cacc_row(j), cacc_col())] = mk(c.r[j], c.i[j]);
 }
 
-__device__ __forceinline__ void cs_half(double y, double x, double &c, double &s)
-{
-    const double t = y / (fabs(x) + sqrt(fma(x, x, y * y)));
-    const double cc = rsqrt(fma(t, t, 1.0));
-    const double ss = t * cc;
-    if (x >= 0) {
-        c = cc;
-        s = ss;
-    } else {
-        c = fabs(ss);
-        s = copysign(cc, y);
-    }
-}
-
 // an optimisation barrier: the N accumulators are complete here, and no
 // memory access moves across it (keeps a product's loads per step bounded)
 template <int N>
@@ -508,8 +494,8 @@ __device__ __forceinline__ void pin_rows(cd *v)
 
 // One round r of the parallel Jacobi sweep: rotations on the pairs (p, q = p ^ r).
 // With a_pq = m w (m = |a_pq|, |w| = 1) the rotation of a pair is
-//   J = [[c, s w], [-s conj(w), c]],  (c, s) = (cos t, sin t), t = atan2(2m, a_pp - a_qq) / 2,
-// which zeroes (J A J^dagger)_pq; its diagonal is real.  It acts on rows,
+//   J = [[c, s w], [-s conj(w), c]],  (c, s) = (cos t, sin t), tan 2t = 2m / (a_pp - a_qq),
+// |t| <= pi/4, which zeroes (J A J^dagger)_pq; its diagonal is real.  It acts on rows,
 // row p <- c row p + s w row q and row q <- c row q - s conj(w) row p, and as
 // J^dagger on columns, column j <- J_jj column j + conj(J_j,j^r) column j ^ r.
 // Lane q takes a_pq from lane p, so both lanes of a pair compute the same
@@ -548,13 +534,21 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
             const int act = L.valid && (m2 > thr2);
             double ja = 1, jbx = 0, jby = 0;
             if (act) {
-                const double ri = rsqrt(m2);
-                double c, sn;
-                cs_half(2 * (m2 * ri), Dg[base + pr] - Dg[base + (pr ^ R)], c, sn);
-                const double swx = sn * (apq.x * ri), swy = sn * (apq.y * ri);
-                ja = c;
-                jbx = isp ? swx : -swx;
-                jby = swy;
+                // the rotation through |theta| <= pi/4 that zeroes a_pq:
+                // tan 2 theta = 2 |a_pq| / d, d = a_pp - a_qq.  With
+                // r = sqrt(d^2 + 4 |a_pq|^2) and D = |d| + r (no cancellation),
+                // c = D / sqrt(D^2 + 4 |a_pq|^2) and s w = sign(d) 2 a_pq / sqrt(D^2 + 4 |a_pq|^2):
+                // two reciprocal square roots, no division, and c^2 + |s w|^2 = 1
+                // to rounding whatever the error of r
+                const double d = Dg[base + pr] - Dg[base + (pr ^ R)];
+                const double m4 = 4 * m2;
+                const double X = fma(d, d, m4);
+                const double D = fabs(d) + X * rsqrt(X);
+                const double q = rsqrt(fma(D, D, m4));
+                const double tq = copysign(2 * q, d);
+                ja = D * q;
+                jbx = isp ? tq * apq.x : -(tq * apq.x);
+                jby = tq * apq.y;
             }
             Ja[base + L.i] = mk(ja, 0);
             Jb[base + L.i] = mk(jbx, jby);
