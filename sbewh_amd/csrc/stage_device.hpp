@@ -477,6 +477,16 @@ __device__ __forceinline__ void cacc_store(cd *lds, const CAcc &c)
     for (int j = 0; j < 4; j++) lds[sw<16>(cacc_row(j), cacc_col())] = mk(c.r[j], c.i[j]);
 }
 
+// 1/sqrt(x) for a positive normal x: the hardware estimate and one
+// second-order correction y (1 + e/2 + 3 e^2/8), e = 1 - x y^2 (the device
+// library's rsqrt without its handling of zero, infinity and NaN)
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // an optimisation barrier: the N accumulators are complete here, and no
 // memory access moves across it (keeps a product's loads per step bounded)
 template <int N>
@@ -498,12 +508,11 @@ __device__ __forceinline__ void pin_rows(cd *v)
 // |t| <= pi/4, which zeroes (J A J^dagger)_pq; its diagonal is real.  It acts on rows,
 // row p <- c row p + s w row q and row q <- c row q - s conj(w) row p, and as
 // J^dagger on columns, column j <- J_jj column j + conj(J_j,j^r) column j ^ r.
-// Lane q takes a_pq from lane p, so both lanes of a pair compute the same
-// rotation from the same operands.  Each stores its row's coefficients
-// (J_jj, J_j,j^r) under index j (the identity when the pair does not
-// rotate), so every lane reads its row's and its columns' coefficients
-// without selects or divergent branches; an identity coefficient pair
-// reproduces its operand exactly.
+// The lane of row p that holds a_pq computes the rotation and stores both
+// rows' coefficients (J_jj, J_j,j^r) under index j (the identity when the
+// pair does not rotate), so every lane reads its row's and its columns'
+// coefficients without selects or divergent branches; an identity
+// coefficient pair reproduces its operand exactly.
 template <int N, int LR, int R>
 __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double thr, double *Dg, cd *Ja, cd *Jb,
                                           int *Fa, int &anyrot)
@@ -513,23 +522,21 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
     constexpr int rc = R % C, rh = R / C;
     constexpr int mrow = R * LR + rh;   // lane mask of the row partner (row i ^ R)
     const int base = L.kk * N;
-    // (a) rotation of the pair (p, q = p ^ R).  Lanes h == rh of rows p and q
-    // hold elements (p, q) and (q, p) = conj(a_pq) in register rc.
+    // (a) rotation of the pair (p, q = p ^ R), p < q: lane h == rh of row p
+    // holds a_pq in register rc
     {
-        constexpr int mpair = R * LR;
         const bool isp = L.i < (L.i ^ R);
         const cd e = A[rc];
         const double thr2 = thr * thr;
+        const bool own = L.h == rh && isp;
         // a round in which no pair of the wavefront rotates leaves A and U
         // unchanged: skip it before computing any rotation (every exchange
         // runs in all lanes: a DPP read from a lane that is switched off
         // returns 0)
-        if (!__any(L.h == rh && isp && L.valid && fma(e.x, e.x, e.y * e.y) > thr2)) return;
+        if (!__any(own && L.valid && fma(e.x, e.x, e.y * e.y) > thr2)) return;
         anyrot = 1;
-        const cd o = xchg<mpair>(e);
-        const cd apq = isp ? e : o;
-        const int pr = isp ? L.i : (L.i ^ R);
-        if (L.h == rh) {
+        if (own) {
+            const cd apq = e;
             const double m2 = fma(apq.x, apq.x, apq.y * apq.y);
             const int act = L.valid && (m2 > thr2);
             double ja = 1, jbx = 0, jby = 0;
@@ -540,19 +547,22 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
                 // c = D / sqrt(D^2 + 4 |a_pq|^2) and s w = sign(d) 2 a_pq / sqrt(D^2 + 4 |a_pq|^2):
                 // two reciprocal square roots, no division, and c^2 + |s w|^2 = 1
                 // to rounding whatever the error of r
-                const double d = Dg[base + pr] - Dg[base + (pr ^ R)];
+                const double d = Dg[base + L.i] - Dg[base + (L.i ^ R)];
                 const double m4 = 4 * m2;
                 const double X = fma(d, d, m4);
-                const double D = fabs(d) + X * rsqrt(X);
-                const double q = rsqrt(fma(D, D, m4));
+                const double D = fabs(d) + X * rsqrt_pos(X);
+                const double q = rsqrt_pos(fma(D, D, m4));
                 const double tq = copysign(2 * q, d);
                 ja = D * q;
-                jbx = isp ? tq * apq.x : -(tq * apq.x);
+                jbx = tq * apq.x;
                 jby = tq * apq.y;
             }
+            // row p: (c, s w); row q: (c, -s conj(w))
             Ja[base + L.i] = mk(ja, 0);
             Jb[base + L.i] = mk(jbx, jby);
-            if (isp) Fa[base + L.i] = act;
+            Ja[base + (L.i ^ R)] = mk(ja, 0);
+            Jb[base + (L.i ^ R)] = mk(-jbx, jby);
+            Fa[base + L.i] = act;
         }
     }
     wsync();
