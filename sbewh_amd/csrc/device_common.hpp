@@ -139,12 +139,28 @@ __device__ __forceinline__ double xor_dpp(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// lane ^ MASK exchange (MASK < 64) through the LDS crossbar (ds_bpermute,
+// no LDS storage); l4 = 4 x this lane's index in the wavefront.  One address
+// for the four dwords and no range check (every lane ^ MASK is a lane of the
+// wavefront).
+template <int MASK>
+__device__ __forceinline__ cd xchg_bp(cd v, int l4)
+{
+    static_assert(MASK > 0 && MASK < 64, "lane mask within one wavefront");
+    const int addr = l4 ^ (MASK << 2);
+    const long long bx = __double_as_longlong(v.x), by = __double_as_longlong(v.y);
+    const int xl = __builtin_amdgcn_ds_bpermute(addr, (int)bx), xh = __builtin_amdgcn_ds_bpermute(addr, (int)(bx >> 32));
+    const int yl = __builtin_amdgcn_ds_bpermute(addr, (int)by), yh = __builtin_amdgcn_ds_bpermute(addr, (int)(by >> 32));
+    return cd{__longlong_as_double(((long long)xh << 32) | (unsigned int)xl),
+              __longlong_as_double(((long long)yh << 32) | (unsigned int)yl)};
+}
+
 // lane ^ MASK exchange of a complex value: DPP for MASK < 16, ds_bpermute otherwise
 template <int MASK>
-__device__ __forceinline__ cd xchg(cd v)
+__device__ __forceinline__ cd xchg(cd v, int l4)
 {
     if constexpr (MASK < 16) return cd{xor_dpp<MASK>(v.x), xor_dpp<MASK>(v.y)};
-    else return cd{__shfl_xor(v.x, MASK), __shfl_xor(v.y, MASK)};
+    else return xchg_bp<MASK>(v, l4);
 }
 
 // sum over aligned segments of `seg` lanes (seg a power of two <= 64)

@@ -41,6 +41,7 @@ struct Geo {
 
 struct Lane {
     int kk, i, h;
+    int l4;  // 4 x the lane's index in the wavefront (a ds_bpermute address)
     long k;
     bool valid;
 };
@@ -55,6 +56,7 @@ __device__ __forceinline__ Lane lane_id(long nk, long blk)
     int li = lane % G::LK;
     L.i = li / LR;
     L.h = li % LR;
+    L.l4 = (lane & 63) << 2;
     L.k = blk * G::KPW + L.kk;
     L.valid = L.k < nk;
     return L;
@@ -575,10 +577,10 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         if (c2 < c) continue;
         // row update of the register pair (c, c2); the partner row holds them in (c2, c)
         cd a0 = A[c], a1 = A[c2];
-        const cd b0 = xchg<mrow>(A[c2]), b1 = xchg<mrow>(A[c]);
+        const cd b0 = xchg<mrow>(A[c2], L.l4), b1 = xchg<mrow>(A[c], L.l4);
         a0 = cfma(cb, b0, cscale(a0, ca));
         if constexpr (!jac_ut<N, LR>()) {
-            const cd v0 = xchg<mrow>(U[c2]), v1 = xchg<mrow>(U[c]);
+            const cd v0 = xchg<mrow>(U[c2], L.l4), v1 = xchg<mrow>(U[c], L.l4);
             U[c] = cfma(cb, v0, cscale(U[c], ca));
             if (c2 != c) U[c2] = cfma(cb, v1, cscale(U[c2], ca));
         }
@@ -593,8 +595,8 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
             d0 = a1;
             d1 = a0;
         } else {
-            d0 = xchg<rh>(a1);
-            d1 = xchg<rh>(a0);
+            d0 = xchg<rh>(a1, L.l4);
+            d1 = xchg<rh>(a0, L.l4);
         }
         const int j0 = L.i ^ (L.h * C + c);
         const double ja = Ja[base + j0].x;
@@ -607,8 +609,8 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
                 e0 = u1;
                 e1 = u0;
             } else {
-                e0 = xchg<rh>(u1);
-                e1 = xchg<rh>(u0);
+                e0 = xchg<rh>(u1, L.l4);
+                e1 = xchg<rh>(u0, L.l4);
             }
             U[c] = cfma(jb, e0, cscale(u0, ja));
             if (c2 != c) {
@@ -668,7 +670,7 @@ __device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sw
         // re-materialise the lane coordinates each sweep so that the per-round
         // LDS addresses are not hoisted out of the sweep loop (register pressure)
         Lane L = L0;
-        asm volatile("" : "+v"(L.i), "+v"(L.h), "+v"(L.kk));
+        asm volatile("" : "+v"(L.i), "+v"(L.h), "+v"(L.kk), "+v"(L.l4));
         double s = 0;
 #pragma unroll
         for (int c = 0; c < C; c++) {
