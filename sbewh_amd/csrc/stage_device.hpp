@@ -324,41 +324,29 @@ __device__ __forceinline__ void load_xrow(const cd *lds, const Lane &L, cd *v)
     for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i, L.i ^ (L.h * G::C + c))];
 }
 
-// The Jacobi's eigenvector registers.  With jac_ut, lane (i, h) register c
-// holds U[i ^ (h C + c)][i] (the xrow layout of U^T): a rotation of rows
+// The Jacobi's eigenvector registers: lane (i, h) register c holds
+// U[i ^ (h C + c)][i] (the xrow layout of U^T).  A rotation of rows
 // (j, j ^ r) of U then combines register c with register c ^ (r mod C) of the
 // same lane (r < C) or of lane h ^ (r / C) in the row's own lane group, and it
 // uses the coefficients the column update of A reads anyway, instead of an
-// exchange with the partner row's lanes.  Every layout uses it (for 16 bands
-// at 4 lanes per row it replaces the U half of the ds_bpermute exchanges of
-// rounds 4-15 by register selects: 2.14 ms per 48^3 launch against 2.20 with
-// U kept in rows).
-template <int N, int LR>
-constexpr bool jac_ut() { return true; }
-
-// U registers (either layout) <-> LDS natural layout
+// exchange with the partner row's lanes (for 16 bands at 4 lanes per row this
+// replaces the U half of the ds_bpermute exchanges of rounds 4-15 by DPP moves
+// inside the lane group: 2.14 ms per 48^3 launch against 2.20 with U kept in
+// rows).  U registers <-> LDS natural layout:
 template <int N, int LR>
 __device__ __forceinline__ void store_u(cd *lds, const Lane &L, const cd *v)
 {
     using G = Geo<N, LR>;
-    if constexpr (jac_ut<N, LR>()) {
 #pragma unroll
-        for (int c = 0; c < G::C; c++) lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)] = v[c];
-    } else {
-        store_xrow<N, LR>(lds, L, v);
-    }
+    for (int c = 0; c < G::C; c++) lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)] = v[c];
 }
 
 template <int N, int LR>
 __device__ __forceinline__ void load_u(const cd *lds, const Lane &L, cd *v)
 {
     using G = Geo<N, LR>;
-    if constexpr (jac_ut<N, LR>()) {
 #pragma unroll
-        for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)];
-    } else {
-        load_xrow<N, LR>(lds, L, v);
-    }
+    for (int c = 0; c < G::C; c++) v[c] = lds[sw<N>(L.i ^ (L.h * G::C + c), L.i)];
 }
 
 // padded rows and columns (index >= n) of U are the identity
@@ -378,12 +366,10 @@ __device__ __forceinline__ void u_pad_identity(const Lane &L, int n, cd *v)
 template <int N, int LR>
 __device__ __forceinline__ void u_rows(cd *lds, const Lane &L, cd *v)
 {
-    if constexpr (jac_ut<N, LR>()) {
-        store_u<N, LR>(lds, L, v);
-        wsync();
-        load_xrow<N, LR>(lds, L, v);
-        wsync();
-    }
+    store_u<N, LR>(lds, L, v);
+    wsync();
+    load_xrow<N, LR>(lds, L, v);
+    wsync();
 }
 
 // ---- 16 x 16 complex products on the FP64 matrix cores (16-band layouts) --
@@ -579,11 +565,6 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         cd a0 = A[c], a1 = A[c2];
         const cd b0 = xchg<mrow>(A[c2], L.l4), b1 = xchg<mrow>(A[c], L.l4);
         a0 = cfma(cb, b0, cscale(a0, ca));
-        if constexpr (!jac_ut<N, LR>()) {
-            const cd v0 = xchg<mrow>(U[c2], L.l4), v1 = xchg<mrow>(U[c], L.l4);
-            U[c] = cfma(cb, v0, cscale(U[c], ca));
-            if (c2 != c) U[c2] = cfma(cb, v1, cscale(U[c2], ca));
-        }
         if (c2 != c) {
             a1 = cfma(cb, b1, cscale(a1, ca));
         } else {
@@ -601,7 +582,7 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         const int j0 = L.i ^ (L.h * C + c);
         const double ja = Ja[base + j0].x;
         const cd jb = Jb[base + j0];
-        if constexpr (jac_ut<N, LR>()) {
+        {
             // U^T registers: row j0 of U (register c) pairs with row j0 ^ R (register c2 of lane h ^ rh)
             const cd u0 = U[c], u1 = U[c2];
             cd e0, e1;
