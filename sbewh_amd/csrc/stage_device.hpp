@@ -199,9 +199,40 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
             tp[u] = Tb + ph.y;
             acc[u] = mk(0, 0);
         }
+        if constexpr (N > 4) {
 #pragma unroll
-        for (int q = 0; q < WQ; q++) {
-            if (q < WL) {
+            for (int q = 0; q < WQ; q++) {
+                if (q < WL) {
+#pragma unroll
+                    for (int u = 0; u < CH; u++) {
+                        const double2 tq = tp[u][q], yq = yp[u][(long)q * M];
+                        acc[u] = cfma(mk(tq.x, tq.y), mk(yq.x, yq.y), acc[u]);
+                    }
+                }
+            }
+        } else {
+            // few bands, registers to spare: QG cells of the last axis per
+            // trip, their 2 QG CH loads in flight together (one latency per
+            // trip instead of one per cell), the same products added in the
+            // same order
+            constexpr int QG = 8 / CH > 1 ? 8 / CH : 1;
+            int q = 0;
+            for (; q + QG <= WL; q += QG) {
+                double2 tq[QG][CH], yq[QG][CH];
+#pragma unroll
+                for (int t = 0; t < QG; t++)
+#pragma unroll
+                    for (int u = 0; u < CH; u++) {
+                        tq[t][u] = tp[u][q + t];
+                        yq[t][u] = yp[u][(long)(q + t) * M];
+                    }
+#pragma unroll
+                for (int t = 0; t < QG; t++)
+#pragma unroll
+                    for (int u = 0; u < CH; u++)
+                        acc[u] = cfma(mk(tq[t][u].x, tq[t][u].y), mk(yq[t][u].x, yq[t][u].y), acc[u]);
+            }
+            for (; q < WL; q++) {
 #pragma unroll
                 for (int u = 0; u < CH; u++) {
                     const double2 tq = tp[u][q], yq = yp[u][(long)q * M];
@@ -218,11 +249,78 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
     });
 }
 
+// stage input rho = x + sum_j c_j k_j with the number of derivatives NT
+// fixed at compile time, so that the loads of x and of every k_j for a chunk
+// of entries are issued before the first combination (one memory latency per
+// chunk instead of one per derivative); the same sums in the same order as
+// load_rho.  For few bands, where the registers for (NT + 1) x CH loads in
+// flight are free.
+template <int N, int LR, class Lay, int NT>
+__device__ __forceinline__ void load_rho_nt(cd *lds, const StageArgs &a, long blk)
+{
+    using G = Geo<N, LR>;
+    const int n = a.n;
+    const StageInput &in = a.in;
+    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
+    constexpr int CH = Lay::CHUNK < IT ? Lay::CHUNK : IT;
+    const double2 *src[NT + 1];
+    double cf[NT + 1];
+    src[0] = reinterpret_cast<const double2 *>(in.x);
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        src[j + 1] = reinterpret_cast<const double2 *>(in.k[j]);
+        cf[j + 1] = in.c[j];
+    }
+    for_chunks<IT, CH>([&](auto i0c) {
+        constexpr int i0 = decltype(i0c)::value;
+        long idx_[CH];
+        bool ok[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int kk = e / G::NN, idx = e % G::NN;
+            const int r = idx / N, c = idx % N;
+            const long k = blk * G::KPW + kk;
+            ok[u] = r < n && c < n && k < a.nk;
+            // masked entries read state entry 8 (valid) and are zeroed at the end
+            idx_[u] = ok[u] ? 8 + k * a.P + r * n + c : 8;
+        }
+        double2 v[NT + 1][CH];
+#pragma unroll
+        for (int j = 0; j <= NT; j++)
+#pragma unroll
+            for (int u = 0; u < CH; u++) v[j][u] = src[j][idx_[u]];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            double re = v[0][u].x, im = v[0][u].y;
+#pragma unroll
+            for (int j = 1; j <= NT; j++) {
+                re += cf[j] * v[j][u].x;
+                im += cf[j] * v[j][u].y;
+            }
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int idx = e % G::NN;
+            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? mk(re, im) : mk(0, 0);
+        }
+    });
+}
+
 // stage input rho = x + sum_j c_j k_j (the Dormand-Prince stage combination,
 // left to right as in the reference's scale_sum) into LDS natural layout
 template <int N, int LR, class Lay = SwLay<N>>
 __device__ __forceinline__ void load_rho(cd *lds, const StageArgs &a, long blk)
 {
+    if constexpr (N <= 4) {
+        switch (a.in.nt) {
+        case 0: load_rho_nt<N, LR, Lay, 0>(lds, a, blk); return;
+        case 1: load_rho_nt<N, LR, Lay, 1>(lds, a, blk); return;
+        case 2: load_rho_nt<N, LR, Lay, 2>(lds, a, blk); return;
+        case 3: load_rho_nt<N, LR, Lay, 3>(lds, a, blk); return;
+        case 4: load_rho_nt<N, LR, Lay, 4>(lds, a, blk); return;
+        case 5: load_rho_nt<N, LR, Lay, 5>(lds, a, blk); return;
+        default: load_rho_nt<N, LR, Lay, 6>(lds, a, blk); return;
+        }
+    }
     using G = Geo<N, LR>;
     const int n = a.n;
     const StageInput &in = a.in;
