@@ -44,6 +44,46 @@ def test_jacobi_batch(n):
     assert np.abs(np.sort(e, 1) - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("n", [2, 8, 16])
+def test_jacobi_batch_degenerate_and_diagonal(n):
+    """the rotation rule at its edges: exactly degenerate eigenvalues (pairs
+    with a_pp - a_qq = 0 or -0.0 and a non-zero a_pq), a diagonal matrix with
+    equal entries and tiny off-diagonal couplings, and a matrix that is
+    already diagonal (nothing rotates): U unitary, U H U^dagger diagonal, the
+    eigenvalues those of numpy"""
+    say(f"[gpu] jacobi edge cases n={n}")
+    rng = np.random.default_rng(300 + n)
+    mats = []
+    # repeated eigenvalues in a random basis
+    for _ in range(40):
+        ev = np.repeat(rng.normal(size=(n + 1) // 2), 2)[:n]
+        Q, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+        mats.append(Q @ np.diag(ev) @ np.conj(Q.T))
+    # equal diagonal, tiny couplings (d = 0 exactly in the first rotations), and -0.0 differences
+    for s in (1e-3, 1e-9):
+        H = np.eye(n, dtype=complex) * 0.7
+        H += s * cases.random_hermitian(rng, 1, n)[0] * (1 - np.eye(n))
+        mats.append(H)
+        Hm = H.copy()
+        Hm[np.diag_indices(n)] = np.where(np.arange(n) % 2 == 1, -0.0, 0.0)
+        mats.append(Hm)
+    # already diagonal
+    mats.append(np.diag(rng.normal(size=n)).astype(complex))
+    H = np.array(mats)
+    U, e = jacobi_batch(H, 12)
+    I = np.eye(n)
+    assert np.abs(U @ np.conj(np.swapaxes(U, 1, 2)) - I).max() < 1e-13
+    D = U @ H @ np.conj(np.swapaxes(U, 1, 2))
+    off = D - np.einsum("kii->ki", D)[:, :, None] * I
+    scale = np.abs(H).max(axis=(1, 2))
+    assert (np.abs(off).max(axis=(1, 2)) < 1e-12 * scale).all()
+    ref = np.linalg.eigvalsh(H)
+    assert (np.abs(np.sort(e, 1) - ref).max(axis=1) < 1e-12 * scale).all()
+    # a diagonal input comes back unchanged (no rotation)
+    assert np.array_equal(e[-1], H[-1].diagonal().real)
+    assert np.array_equal(U[-1], np.eye(n))
+
+
 @pytest.mark.parametrize("name", ["chain", "square", "cubic"])
 def test_phase_sum(name):
     say(f"[gpu] phase sum {name}")
