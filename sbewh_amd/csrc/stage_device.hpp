@@ -616,9 +616,9 @@ __device__ __forceinline__ void jac_round(cd *A, cd *U, const Lane &L, double th
         const double thr2 = thr * thr;
         const bool own = L.h == rh && isp;
         // a round in which no pair of the wavefront rotates leaves A and U
-        // unchanged: skip it before computing any rotation (every exchange
-        // runs in all lanes: a DPP read from a lane that is switched off
-        // returns 0)
+        // unchanged: skip it before computing any rotation (the test is
+        // wave-uniform, so the exchanges below always run in all lanes: a DPP
+        // read from a lane that is switched off returns 0)
         if (!__any(own && L.valid && fma(e.x, e.x, e.y * e.y) > thr2)) return;
         anyrot = 1;
         if (own) {
@@ -731,8 +731,9 @@ __device__ __forceinline__ void jac_sweep(std::integer_sequence<int, Rm1...>, cd
     (jac_round<N, LR, Rm1 + 1>(A, U, L, thr, Dg, Ja, Jb, Fa, anyrot), ...);
 }
 
-// Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (rows), eigenvalues
-// in Dg[kk*N + i].  LDS scratch: Dg[KPW*N], Ja/Jb[KPW*N] (rotation coefficients), Fa[KPW*N].
+// Wavefront Jacobi on the xrow matrix A (Hermitian) -> U (the transposed
+// register layout of store_u / load_u), eigenvalues in Dg[kk*N + i].  LDS
+// scratch: Dg[KPW*N], Ja/Jb[KPW*N] (rotation coefficients), Fa[KPW*N].
 template <int N, int LR>
 __device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sweeps, double *Dg, cd *Ja, cd *Jb, int *Fa,
                             bool initU = true)
