@@ -1204,8 +1204,9 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     }
 
     cd dr[C];
+    cd U[C];  // the eigenvectors, kept in registers from the Jacobi to the dephasing
     if (a.dephase) {
-        cd A[C], U[C];
+        cd A[C];
         if (a.warm) {
             // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
             load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
@@ -1230,9 +1231,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
         }
         store_u<N, LR>(s1, L, U);  // U -> S1
         wsync();
-        // the eigenvectors go to the per-k cache: warm start of the next call,
-        // and the copy the dephasing below reloads (each lane reads back only
-        // the entries it stored itself, same-address order within a thread)
+        // the eigenvectors go to the per-k cache: warm start of the next call
         store_ucache<N, LR>(a, S1, blk);
     }
     load_rho<N, LR>(S0, a, blk);  // rho -> S0
@@ -1253,7 +1252,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     if (a.dephase) {
         // dephasing in the eigenbasis: U^dagger (D o (U rho U^dagger)) U
         wsync();
-        load_mats<N, LR>(S1, a.ucache, nn, 0, n, a.nk, blk);  // U -> S1
+        store_u<N, LR>(s1, L, U);  // U -> S1 (padded rows and columns: the identity)
         wsync();
         cd T[C];
         lds_product<N, LR, 0, 0>(s1, s0, L, T);  // T = U rho
