@@ -198,7 +198,9 @@ __device__ __forceinline__ void jk_pass(const ObsArgs &o, long k0, long kend, cd
 template <int N, int LR>
 __device__ __noinline__ void stage_call(const StageArgs *sa, long blk)
 {
-    rhs_stage_body<N, LR>(*sa, blk);
+    // H_E formed separately: the paired form's registers count against this
+    // call's frame inside the resident kernel
+    rhs_stage_body<N, LR, false>(*sa, blk);
 }
 
 template <int N, int LR>

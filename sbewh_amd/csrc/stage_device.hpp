@@ -249,6 +249,97 @@ __device__ __forceinline__ void phase_mats(cd *lds, const StageArgs &a, int op, 
     });
 }
 
+// Few bands (n <= 4): H_0 into LDS and H_E into registers in one pass over
+// the phase-sum cells, so that H_E's loads ride in the trips of H_0's (the
+// same sums in the same order as two phase_mats calls).  he[i] is the lane's
+// entry i in phase_mats' lane -> entry map (zero where masked); false, with
+// nothing done, for an R box wider than 16 cells along the last axis.
+template <int N, int LR, class Lay = SwLay<N>>
+__device__ __forceinline__ bool phase_mats_pair(cd *lds, const StageArgs &a, long blk, cd *he)
+{
+    static_assert(N <= 4, "registers for both operators");
+    using G = Geo<N, LR>;
+    constexpr int IT = G::KPW * G::NN / 64;  // entries per lane
+    const int n = a.n, nn = n * n, M = 2 * nn, WL = a.ph.WL;
+    if (WL > 16) return false;
+    const int2 *kph = reinterpret_cast<const int2 *>(a.ph.kphase);
+    const double2 *Yb = reinterpret_cast<const double2 *>(a.ph.Y);
+    const double2 *Tb = reinterpret_cast<const double2 *>(a.ph.twL);
+    constexpr int CH = Lay::CHUNK < IT ? Lay::CHUNK : IT;
+    constexpr int QG = 8 / CH > 1 ? 8 / CH : 1;
+    for_chunks<IT, CH>([&](auto i0c) {
+        constexpr int i0 = decltype(i0c)::value;
+        cd acc0[CH], acc1[CH];
+        const double2 *yp[CH];
+        const double2 *tp[CH];
+        bool ok[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int kk = e / G::NN, idx = e % G::NN;
+            const int r = idx / N, c = idx % N;
+            const long k = blk * G::KPW + kk;
+            const bool krc = r < n && c < n;
+            ok[u] = krc && k < a.nk;
+            const int2 ph = kph[k < a.nk ? k : 0];
+            yp[u] = Yb + (long)ph.x * M + (krc ? r * n + c : 0);  // operator 0; operator 1 at + nn
+            tp[u] = Tb + ph.y;
+            acc0[u] = mk(0, 0);
+            acc1[u] = mk(0, 0);
+        }
+        int q = 0;
+        for (; q + QG <= WL; q += QG) {
+            double2 tq[QG][CH], y0[QG][CH], y1[QG][CH];
+#pragma unroll
+            for (int t = 0; t < QG; t++)
+#pragma unroll
+                for (int u = 0; u < CH; u++) {
+                    tq[t][u] = tp[u][q + t];
+                    y0[t][u] = yp[u][(long)(q + t) * M];
+                    y1[t][u] = yp[u][(long)(q + t) * M + nn];
+                }
+#pragma unroll
+            for (int t = 0; t < QG; t++)
+#pragma unroll
+                for (int u = 0; u < CH; u++) {
+                    const cd tw = mk(tq[t][u].x, tq[t][u].y);
+                    acc0[u] = cfma(tw, mk(y0[t][u].x, y0[t][u].y), acc0[u]);
+                    acc1[u] = cfma(tw, mk(y1[t][u].x, y1[t][u].y), acc1[u]);
+                }
+        }
+        for (; q < WL; q++) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const double2 tq = tp[u][q], y0 = yp[u][(long)q * M], y1 = yp[u][(long)q * M + nn];
+                acc0[u] = cfma(mk(tq.x, tq.y), mk(y0.x, y0.y), acc0[u]);
+                acc1[u] = cfma(mk(tq.x, tq.y), mk(y1.x, y1.y), acc1[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + 64 * (i0 + u);
+            const int idx = e % G::NN;
+            lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = ok[u] ? acc0[u] : mk(0, 0);
+            he[i0 + u] = ok[u] ? acc1[u] : mk(0, 0);
+        }
+    });
+    return true;
+}
+
+// the lane's entries (phase_mats' lane -> entry map) into LDS natural layout
+template <int N, int LR, class Lay = SwLay<N>>
+__device__ __forceinline__ void store_entries(cd *lds, const cd *v)
+{
+    using G = Geo<N, LR>;
+    constexpr int IT = G::KPW * G::NN / 64;
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+        const int e = threadIdx.x + 64 * i;
+        const int idx = e % G::NN;
+        lds[(e / G::NN) * Lay::MS + Lay::at(idx / N, idx % N)] = v[i];
+    }
+}
+
 // stage input rho = x + sum_j c_j k_j with the number of derivatives NT
 // fixed at compile time, so that the loads of x and of every k_j for a chunk
 // of entries are issued before the first combination (one memory latency per
@@ -1043,7 +1134,7 @@ __device__ __forceinline__ void reg_product(const cd *xr, const cd *s, int i, cd
     }
 }
 
-template <int N>
+template <int N, bool PAIR>
 __device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long blk)
 {
     using G = Geo<N, 1>;
@@ -1061,6 +1152,14 @@ __device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long
 
     cd U[C];   // eigenvectors, columns
     cd T1[C];  // U rho, rows
+    // few bands: H_E formed with H0, kept in registers until it is needed
+    constexpr int HE = N <= 4 ? KPW * G::NN / 64 : 1;
+    cd he[HE];
+    bool he_ready = false;
+    auto form_h0 = [&]() {  // H0 -> S
+        if constexpr (N <= 4 && PAIR) he_ready = phase_mats_pair<N, 1, Lay>(S, a, blk, he);
+        if (!he_ready) phase_mats<N, 1, Lay>(S, a, 0, blk);
+    };
     if (a.dephase) {
         cd A[C];
         if (a.warm) {
@@ -1074,7 +1173,7 @@ __device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long
             u_pad_identity<N, 1>(L, n, T);
             u_pad_identity<N, 1>(L, n, U);
             wsync();
-            phase_mats<N, 1, Lay>(S, a, 0, blk);  // H0 -> S
+            form_h0();
             wsync();
             reg_product<N, 0, 0>(T, s, L.i, A);  // U' H0
             wsync();
@@ -1085,7 +1184,7 @@ __device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long
             reg_product<N, 0, 1>(T, s, L.i, A);  // (U' H0) U'^dagger
             wave_jacobi<N, 1>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
         } else {
-            phase_mats<N, 1, Lay>(S, a, 0, blk);  // H0 -> S
+            form_h0();
             wsync();
             slot_load_rows<N>(s, L.i, A);
             wave_jacobi<N, 1>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
@@ -1110,7 +1209,8 @@ __device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long
         cd rr[C], T[C];
         slot_load_rows<N>(s, L.i, rr);
         wsync();
-        phase_mats<N, 1, Lay>(S, a, 1, blk);  // H_E -> S
+        if (he_ready) store_entries<N, 1, Lay>(S, he);  // H_E -> S
+        else phase_mats<N, 1, Lay>(S, a, 1, blk);
         wsync();
         reg_product<N, 0, 0>(rr, s, L.i, T);  // rho H_E
         wsync();
@@ -1162,8 +1262,11 @@ __device__ __forceinline__ void rhs_stage_body_r1(const StageArgs &a, const long
     }
 }
 
-// the stage for the KPW k-points of logical block blk of one instance
-template <int N, int LR>
+// the stage for the KPW k-points of logical block blk of one instance.
+// PAIR: for n <= 4, H_E is formed together with H_0 and held in registers
+// (fewer memory trips; more registers, which a caller through a call
+// boundary may not want)
+template <int N, int LR, bool PAIR = true>
 __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long blk)
 {
     using G = Geo<N, LR>;
@@ -1172,7 +1275,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
         rhs_stage_body16<LR>(a, blk);
         return;
     } else if constexpr (LR == 1) {
-        rhs_stage_body_r1<N>(a, blk);
+        rhs_stage_body_r1<N, PAIR>(a, blk);
         return;
     } else {
     __shared__ cd S0[G::KPW * G::NN];
@@ -1205,12 +1308,20 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
 
     cd dr[C];
     cd U[C];  // the eigenvectors, kept in registers from the Jacobi to the dephasing
+    // few bands: H_E formed with H_0, kept in registers until it is needed
+    constexpr int HE = N <= 4 ? G::KPW * G::NN / 64 : 1;
+    cd he[HE];
+    bool he_ready = false;
+    auto form_h0 = [&]() {  // H_0 -> S1
+        if constexpr (N <= 4 && PAIR) he_ready = phase_mats_pair<N, LR>(S1, a, blk, he);
+        if (!he_ready) phase_mats<N, LR>(S1, a, 0, blk);
+    };
     if (a.dephase) {
         cd A[C];
         if (a.warm) {
             // warm start: A = U' H0 U'^dagger with U' the previous eigenvectors
             load_mats<N, LR>(S0, a.ucache, nn, 0, n, a.nk, blk);
-            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
+            form_h0();
             wsync();
             load_u<N, LR>(s0, L, U);
             // padded rows and columns of U' are the identity
@@ -1224,7 +1335,7 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
             lds_product<N, LR, 0, 1>(s0, s1, L, A);  // (U' H0) U'^dagger
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa, false);
         } else {
-            phase_mats<N, LR>(S1, a, 0, blk);  // H_0 -> S1
+            form_h0();
             wsync();
             load_xrow<N, LR>(s1, L, A);
             wave_jacobi<N, LR>(A, U, L, a.sweeps, Dg, Ja, Jb, Fa);
@@ -1238,7 +1349,8 @@ __device__ __forceinline__ void rhs_stage_body(const StageArgs &a, const long bl
     wsync();
     if (a.xout) store_state<N, LR>(a.xout, S0, n, a.P, a.nk, blk);
     // coherent part: -i (H_E rho - rho H_E)
-    phase_mats<N, LR>(S1, a, 1, blk);  // H_E -> S1
+    if (he_ready) store_entries<N, LR>(S1, he);  // H_E -> S1
+    else phase_mats<N, LR>(S1, a, 1, blk);
     wsync();
     {
         cd t1[C];
