@@ -842,14 +842,19 @@ __device__ __forceinline__ void wave_jacobi(cd *A, cd *U, const Lane &L0, int sw
         // LDS addresses are not hoisted out of the sweep loop (register pressure)
         Lane L = L0;
         asm volatile("" : "+v"(L.i), "+v"(L.h), "+v"(L.kk), "+v"(L.l4));
-        double s = 0;
+        double s = 0, m2 = 0;
 #pragma unroll
         for (int c = 0; c < C; c++) {
             int j = L.i ^ (L.h * C + c);
             if (L.i <= j) s += fabs(A[c].x) + fabs(A[c].y);
+            if (L.i != j) m2 = fmax(m2, fma(A[c].x, A[c].x, A[c].y * A[c].y));
         }
         s = seg_sum<G::LK>(s);
         const double thr = 1e-15 * (0.5 * s);
+        // no off-diagonal entry above the threshold: no pair of this sweep
+        // rotates (a round without rotations changes nothing), so the sweep
+        // ends here instead of testing its N - 1 rounds one by one
+        if (!__any(L.valid && m2 > thr * thr)) break;
         int anyrot = 0;
         jac_sweep<N, LR>(std::make_integer_sequence<int, N - 1>{}, A, U, L, thr, Dg, Ja, Jb, Fa, anyrot);
         if (!__any(anyrot)) break;
