@@ -90,19 +90,6 @@ __device__ __forceinline__ double *box_of(const ResidentArgs &r, long blk, int s
     return r.Yb + (blk * SET_MAX + set) * r.Y_stride;
 }
 
-__device__ __forceinline__ ModelMerge merge_into(const ResidentArgs &r, const double E[3], long blk, int set)
-{
-    ModelMerge mm{};
-    mm.Hb = r.Hb;
-    mm.Db = r.Db;
-    for (int d = 0; d < 3; d++) mm.E[d] = d < r.dim ? E[d] : 0.0;
-    mm.nbox = r.nbox;
-    mm.nn = r.nn;
-    mm.dim = r.dim;
-    mm.out = box_of(r, blk, set);
-    return mm;
-}
-
 // off_q = (kGlobalShift + A) . a_0 / 2 pi of a stage input, A formed as the
 // twiddle kernel forms it (the stage combination of the special slots, the
 // derivatives this attempt forms taken from ao)
@@ -261,9 +248,26 @@ __global__ __launch_bounds__(64) void resident_kernel(ResidentArgs r)
                 reinterpret_cast<double2 *>(r.tw6 + q * r.tw_stride)[kph[k].y + rr] = make_double2(v.x, v.y);
             }
         }
-        for (long t = threadIdx.x; t < m * nbox2; t += 64) {
-            const int q = q0 + (int)(t / nbox2);
-            merge_body(merge_into(r, Es[q], blk, q), t % nbox2);
+        // the merged boxes of the m sets differ only in E: each model entry
+        // is read once and merged for every set (the sums of the model merge, merge_body)
+        for (long t = threadIdx.x; t < nbox2; t += 64) {
+            const long M = 2L * r.nn;
+            const long b = t / M;
+            const int mi = (int)(t - b * M), op = mi >= r.nn, e = mi - op * r.nn;
+            const double2 h = reinterpret_cast<const double2 *>(r.Hb)[b * r.nn + e];
+            double2 dv[3] = {};
+            if (op)
+                for (int d = 0; d < r.dim; d++)
+                    dv[d] = reinterpret_cast<const double2 *>(r.Db)[((long)d * r.nbox + b) * r.nn + e];
+            for (int q = q0; q < q0 + m; q++) {
+                double re = h.x, im = h.y;
+                if (op)
+                    for (int d = 0; d < r.dim; d++) {
+                        re += Es[q][d] * dv[d].x;
+                        im += Es[q][d] * dv[d].y;
+                    }
+                reinterpret_cast<double2 *>(box_of(r, blk, q))[t] = make_double2(re, im);
+            }
         }
         __threadfence_block();
         __syncthreads();
