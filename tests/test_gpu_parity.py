@@ -125,6 +125,41 @@ def test_fermi_initial_rhs_observe(name):
     assert rel(d_g2, d_o2) < 1e-12, rel(d_g2, d_o2)
 
 
+@pytest.mark.parametrize("lr", [1, 2])
+def test_wide_last_axis_box(lr, monkeypatch):
+    """an R box wider than 16 cells along the last axis (the 1-D chain padded
+    to |R| <= 10 with small hoppings, Fourier grid 32): the stage kernels form
+    H_0 and H_E one entry at a time, and the few-band bodies take their
+    separate-formation branch; RHS against the oracle, cold and warm-started"""
+    say(f"[gpu] wide R box, chain, {lr} lane(s) per row")
+    monkeypatch.setenv("SBEWH_STAGE_LR", str(lr))
+    base = model.chain1d_model()
+    cells = [(R, 0, 0) for R in range(-10, 11)]
+    n = base.num_wann
+    old = {tuple(int(v) for v in c): i for i, c in enumerate(base.cells)}
+    H = np.zeros((len(cells), n, n), complex)
+    D = np.zeros((len(cells), 3, n, n), complex)
+    for i, c in enumerate(cells):
+        if c in old:
+            H[i], D[i] = base.H[old[c]], base.D[old[c]]
+    rng = np.random.default_rng(11)
+    for R in range(5, 11):
+        h = 1e-4 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+        H[cells.index((R, 0, 0))] = h
+        H[cells.index((-R, 0, 0))] = h.conj().T
+    m = model.TBModel(np.array(cells, np.int32), H, D, base.lattice)
+    assert m.hermiticity_error() == 0
+    cfg = dataclasses.replace(cases.chain(sweeps=8), model=m, Nf=(32, 1, 1))
+    g, o = SBEContext(cfg), Oracle(cfg)
+    st = o.initial_state(o.fermi_level(cfg.occupied_bands))
+    x = cases.perturbed_state(st, o.n, o.P, o.num_k, seed=13)
+    for t in (0.3 * o.t_start, 0.28 * o.t_start):
+        assert rel(g.rhs(x, t), o.rhs(x, t)) < 1e-12
+    assert g.stage_tuning()["lanes_per_row"] == lr
+    g.close()
+    o.close()
+
+
 def test_rhs_no_dephasing_step_mask_cold_jacobi():
     say("[gpu] rhs variants")
     import dataclasses
