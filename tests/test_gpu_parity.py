@@ -275,19 +275,21 @@ def test_stage_per_launch_equals_attempt_per_launch(monkeypatch):
     assert np.array_equal(s0, s1)
 
 
-def test_one_block_attempt_kernel(monkeypatch):
-    """A grid of a single stage block: that block is the one that takes the
-    last ticket of the fused attempt kernel and reduces the error norm alone.
-    Same bits as one launch per stage (own step control), and the oracle's
-    step sequence replayed to 1e-10 with matching error norms."""
-    say("[gpu] chain1d 16 k (one stage block): one launch per attempt vs one per stage")
-    cfg = cases.chain(Nk=(16, 1, 1))
+@pytest.mark.parametrize("nk,lr", [(16, 1), (16, 2), (256, 2)])
+def test_one_block_attempt_kernel(nk, lr, monkeypatch):
+    """The fused attempt kernel against one launch per stage on the 1-D chain:
+    on 16 k a grid of a single stage block (that block takes the last ticket
+    and reduces the error norm alone), on 256 k sixteen blocks.  Same bits as
+    one launch per stage (own step control), and the oracle's step sequence
+    replayed to 1e-10 with matching error norms."""
+    say(f"[gpu] chain1d {nk} k, {lr} lane(s) per row: one launch per attempt vs one per stage")
+    cfg = cases.chain(Nk=(nk, 1, 1))
     o = Oracle(cfg)
     mu = o.fermi_level(cfg.occupied_bands)
     st0 = o.initial_state(mu)
     so = st0.copy()
     ro = o.integrate(so, cfg.output_count)
-    monkeypatch.setenv("SBEWH_STAGE_LR", "1")   # the same layout both ways
+    monkeypatch.setenv("SBEWH_STAGE_LR", str(lr))   # the same layout both ways
     res = {}
     for fuse in ("512", "0"):
         monkeypatch.setenv("SBEWH_FUSE_BELOW", fuse)
